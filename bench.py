@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Benchmark: the 2-bit, 4096-channel, 2^20-sample, 2048-DM (0..2047) dedispersion sweep.
+
+One step = one full sweep of the packed stream resident in HBM (stage 1 + stage 2 of
+``spp_sweep_execute``). Prints one JSON line (rank 0) with the driver's contract fields
+plus ``roofline`` (the dominant kernel, k_sweep, timed with HIP events on its stream)
+and ``cpu_baseline`` (the C/OpenMP restatement of kernels.dedisperse_par, timed on a
+bounded DM sample on this host and extrapolated per DM-trial sample).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "DM-trial·samples/s, 4096ch×2048-DM sweep (bit-exact); p50/p99 gulp latency"
+UNIT = "DM-trial·samples/s"
+VALU_ADD_PEAK_T = 78.6   # T adds/s: 256 CU x 4 SIMD x 16 lanes x 2 (packed) x 2.4 GHz
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def read_traffic(path: Path):
+    """HBM bytes per k_sweep dispatch from the committed rocprofv3 PMC summary, if any."""
+    if not path.exists():
+        return None
+    vals = {}
+    for line in path.read_text().splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return int(float(vals["fetch_bytes_per_dispatch"]) + float(vals["write_bytes_per_dispatch"]))
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(packed_dev, cfg, table, row_len, plane, budget_s: float):
+    """Time the C/OpenMP restatement of dedisperse_par on a bounded DM sample.
+
+    Per DM the reference's Filterbank.dedisperse re-reads and serially unpacks the whole
+    file, then runs dedisperse_par; both costs are timed (the unpack on one thread on a
+    64 MiB slice, scaled). The rows computed are also compared with the GPU plane.
+    """
+    from oracle import c_oracle
+
+    packed = packed_dev.cpu().numpy()
+    threads = c_oracle.max_threads()
+    log(f"cpu baseline: unpacking {packed.size / 2**30:.2f} GiB on {threads} threads")
+    stream = c_oracle.unpack(packed, cfg.nbits, "big")
+    # serial unpack rate (numba's unpack kernels are single-threaded loops)
+    c_oracle.set_threads(1)
+    sl = packed[: 64 << 20]
+    t0 = time.perf_counter()
+    c_oracle.unpack(sl, cfg.nbits, "big")
+    t_unpack = (time.perf_counter() - t0) * packed.size / sl.size
+    c_oracle.set_threads(threads)
+    nch = cfg.nchans
+    out = np.zeros(cfg.nsamps, dtype=np.float32)
+    picks = [0]
+    t_rows = []
+    t0 = time.perf_counter()
+    c_oracle.dedisperse(stream, out, table[0], nch, cfg.nsamps, 0)
+    t_rows.append(time.perf_counter() - t0)
+    exact = bool(np.array_equal(out[: row_len[0]], plane[0, : row_len[0]].cpu().numpy()))
+    per_dm = t_rows[0] + t_unpack
+    extra = max(0, min(7, int(budget_s / per_dm) - 1))
+    for i in np.linspace(0, len(table) - 1, extra + 2)[1:].astype(int)[:extra]:
+        t0 = time.perf_counter()
+        c_oracle.dedisperse(stream, out, table[i], nch, cfg.nsamps, 0)
+        t_rows.append(time.perf_counter() - t0)
+        picks.append(int(i))
+        n = int(row_len[i])
+        exact &= bool(np.array_equal(out[:n], plane[i, :n].cpu().numpy()))
+        log(f"cpu baseline: DM row {i} in {t_rows[-1]:.2f} s")
+    samples = float(sum(row_len[i] for i in picks))
+    secs = float(sum(t_rows)) + t_unpack * len(picks)
+    try:
+        model = next(line.split(":", 1)[1].strip() for line in open("/proc/cpuinfo")
+                     if line.startswith("model name"))
+    except (OSError, StopIteration):
+        model = "unknown"
+    return {
+        "value": samples / secs,
+        "unit": UNIT,
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{len(picks)} DM rows {picks} of the 2048, each over the full 2^20-sample stream "
+                   f"(C/OpenMP restatement of kernels.dedisperse_par, {threads} threads, "
+                   f"{np.mean(t_rows):.2f} s/row) plus the serial unpack of the 1 GiB packed file per "
+                   f"DM ({t_unpack:.2f} s, 1 thread) as Filterbank.dedisperse incurs it; "
+                   f"extrapolated per DM-trial sample"),
+        "cpu_model": model,
+        "rows_bit_exact_vs_gpu": exact,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="wall-clock budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
+              "(the sweep shards by DM list across GPUs; see DESIGN.md)", file=sys.stderr)
+        return 1
+
+    import torch
+
+    from sigpyproc3_amd import _lib
+    from sigpyproc3_amd.params import numpy_simd_features, table_digest
+    from sigpyproc3_amd.sweep import SweepPlan
+    from sigpyproc3_amd.synth import cfg_sweep_2bit, generate_packed
+
+    _lib.load()
+    _lib.require_gpu()
+    dev = _lib.device()
+    cfg = cfg_sweep_2bit()
+    dms = np.arange(2048, dtype=np.float64)
+    table = cfg.delays(dms).astype(np.int32)
+    log(f"generating {cfg.nchans} ch x {cfg.nsamps} samples, {cfg.nbits}-bit, on {torch.cuda.get_device_name()}")
+    packed = generate_packed(cfg, device=dev)
+    plan = SweepPlan(table, cfg.nsamps, cfg.nbits, 1)
+    plane = _lib.empty((plan.ndm, plan.max_len), "float32")
+    torch.cuda.synchronize()
+    log(f"plan: {plan.info.ndm_tiles} DM tiles, max spread {plan.info.max_spread}, "
+        f"workspace {plan.info.workspace_bytes / 2**30:.1f} GiB")
+    for _ in range(args.warmup):
+        plan.execute(packed, plane)
+    torch.cuda.synchronize()
+    log(f"warmup done ({args.warmup} steps)")
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1, e2 in ev:
+        e0.record()
+        plan.build(packed)
+        e1.record()
+        plan.run(plane)
+        e2.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    build_ms = np.array([a.elapsed_time(b) for a, b, _ in ev])
+    sweep_ms = np.array([b.elapsed_time(c) for _, b, c in ev])
+    step_ms = np.array([a.elapsed_time(c) for a, _, c in ev])
+    total = plan.total_samples
+    common = int(plan.info.min_len) * plan.ndm
+    sec_per_step = wall / args.steps
+    value = total / sec_per_step
+    log(f"{args.steps} steps: {sec_per_step * 1e3:.2f} ms/step; k_sweep {sweep_ms.mean():.2f} ms, "
+        f"k_build_g {build_ms.mean():.2f} ms")
+
+    adds = float(total) * cfg.nchans
+    achieved_t = adds / (sweep_ms.mean() * 1e-3) / 1e12
+    traffic = read_traffic(ROOT / "profiles" / "k_sweep_cfg2_traffic.txt")
+    roofline = {
+        "bound": "valu",
+        "achieved": round(achieved_t, 3),
+        "peak": VALU_ADD_PEAK_T,
+        "unit": "Tadd/s",
+        "frac": round(achieved_t / VALU_ADD_PEAK_T, 4),
+        "traffic": traffic,
+        "kernel": "k_sweep",
+        "kernel_ms": round(float(sweep_ms.mean()), 3),
+        "algorithmic_work": f"{total} DM-trial samples x {cfg.nchans} channel adds = {adds:.4e} adds per launch",
+    }
+    if traffic is not None:
+        roofline["hbm_gbs"] = round(traffic / (sweep_ms.mean() * 1e-3) / 1e9, 1)
+        roofline["hbm_frac"] = round(roofline["hbm_gbs"] / HBM_PEAK_GBS, 4)
+    stage1_bytes = plan.nsamps * cfg.nchans * cfg.nbits // 8 + 4 * cfg.nchans * (-(-cfg.nsamps // 64) * 64)
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": UNIT,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec_per_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": "2-bit packed filterbank, 4096 ch x 2^20 samples (64 us), DM 0..2047 step 1 "
+                        "(2048 trials), float32 DM x time plane",
+            "nchans": cfg.nchans, "nsamps": cfg.nsamps, "nbits": cfg.nbits, "ndm": plan.ndm,
+            "fch1": cfg.fch1, "foff": cfg.foff, "tsamp": cfg.tsamp,
+            "dm_trial_samples_ragged": total, "dm_trial_samples_common": common,
+            "delay_table_sha1": table_digest(table), "numpy_simd": numpy_simd_features(),
+        },
+        "latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99)),
+                       "unit": "one 2^20-sample gulp (67.1 s of data) through the sweep"},
+        "stages_ms": {"k_build_g": round(float(build_ms.mean()), 3), "k_sweep": round(float(sweep_ms.mean()), 3),
+                      "k_build_g_gbs": round(stage1_bytes / (build_ms.mean() * 1e-3) / 1e9, 1)},
+        "common_length_value": common / sec_per_step,
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(packed, cfg, table, plan.row_len, plane, args.cpu_seconds)
+        cb = result["cpu_baseline"]
+        result["speedup_vs_cpu_baseline"] = value / cb["value"] if cb["value"] else None
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

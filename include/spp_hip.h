@@ -1,0 +1,171 @@
+/*
+ * libspp_hip -- C ABI of the MI355X (gfx950) filterbank dedispersion hot path.
+ *
+ * Conventions
+ *   - Every entry point returns an int status: SPP_OK (0) on success, SPP_EINVAL for
+ *     an invalid argument (the Python layer raises ValueError), SPP_EHIP for a HIP
+ *     runtime failure (RuntimeError). spp_last_error() returns the thread-local
+ *     message of the last failure on the calling thread.
+ *   - Array arguments are DEVICE pointers unless the name ends in _host. The caller
+ *     owns every buffer; the library allocates no device memory.
+ *   - `stream` is a hipStream_t (NULL = the default stream). Work is enqueued and the
+ *     call returns without synchronising, except where a comment says otherwise.
+ *   - Reference paths are relative to the sigpyproc source tree (src/sigpyproc).
+ */
+#ifndef SPP_HIP_H
+#define SPP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPP_OK 0
+#define SPP_EINVAL 1
+#define SPP_EHIP 2
+
+/* Thread-local message of the last failed call on this thread ("" if none). */
+const char* spp_last_error(void);
+
+/* ABI version of this library. */
+int spp_version(void);
+
+/*
+ * Unpack 1/2/4-bit samples packed in bytes to one byte per sample.
+ * Replaces kernels.unpack{1,2,4}_8_{big,little} (core/kernels.py:20-69) as called by
+ * bits.unpack (io/bits.py:19-70, getattr at 68-69). big != 0: the first sample of a
+ * byte is in its most significant bits. out holds nbytes * (8 / nbits) bytes.
+ */
+int spp_unpack(const uint8_t* in, int64_t nbytes, uint8_t* out, int nbits, int big,
+               void* stream);
+
+/*
+ * Dedisperse one gulp to a time series, unpacking in registers.
+ * Replaces kernels.dedisperse / kernels.dedisperse_par (core/kernels.py:340-362, 542)
+ * as called by Filterbank.dedisperse (base.py:405-420), fused with the unpack of
+ * FileReader.creadinto (io/fileio.py:284-292).
+ *   in      gulp in file order [nsamps][nchans]: packed nbits in {1,2,4,8} with bit
+ *           order `big`, or float32 samples for nbits == 32
+ *   delays     int32 [nchans]
+ *   max_delay  max_c |delays[c]| (the reference computes it inside the kernel,
+ *              kernels.py:356; here the caller, which holds the host table, passes it)
+ *   out        out[out_start + t] = sum_c in[t + delays[c]][c] for t < nsamps - max_delay;
+ *              a negative row index t + d wraps to t + d + nsamps (numba semantics).
+ * Integer data is summed exactly; float32 data is summed in float64 in channel order.
+ * Rows outside [0, nsamps) after the wrap read as 0, so no delay value reads outside
+ * the gulp.
+ */
+int spp_dedisperse(const void* in, int nbits, int big, int64_t nchans, int64_t nsamps,
+                   const int32_t* delays, int64_t max_delay, float* out, int64_t out_start,
+                   void* stream);
+
+/*
+ * Subband one gulp. Replaces kernels.subband (core/kernels.py:379-401) as called by
+ * Filterbank.subband (base.py:1096-1105):
+ *   out[t*nsubs + chan_to_sub[c]] += in[t + delays[c]][c]  for t < nsamps - maxdelay.
+ * The entry point first zeroes out[0 : min(out_size, (nsamps - maxdelay) * nsubs)]
+ * (the reference's np.empty buffer is never zeroed, base.py:1073; zeroing per gulp is
+ * the kernel semantics its comment asks for). Writes at flat indices >= out_size are
+ * dropped. nbits in {1,2,4,8} (packed, bit order `big`) or 32 (float32).
+ */
+int spp_subband(const void* in, int nbits, int big, int64_t nchans, int64_t nsamps,
+                const int32_t* delays, const int32_t* chan_to_sub, int64_t maxdelay,
+                int64_t nsubs, float* out, int64_t out_size, void* stream);
+
+/*
+ * DM-trial sweep: plane[i][t] = Filterbank.dedisperse(dm_i).data[t] (base.py:360-424)
+ * for every row i of a host delay table, over a whole stream resident on the device.
+ * No single reference function: it is the stack over DMs of kernels.dedisperse
+ * (core/kernels.py:348-362) outputs, which do not depend on the gulp schedule for
+ * non-negative delays. Rows have length row_len[i] = nsamps - max_c delays[i][c];
+ * plane entries in [row_len[i], ceil(max_len/1024)*1024) are written as 0.
+ */
+typedef struct spp_sweep_plan spp_sweep_plan;
+
+typedef struct {
+    int64_t workspace_bytes; /* device workspace the caller must provide           */
+    int64_t max_len;         /* longest row (the plane's minimum leading dimension) */
+    int64_t min_len;         /* shortest row                                        */
+    int64_t total_samples;   /* sum of row lengths (DM-trial samples)               */
+    int64_t ndm_tiles;       /* DM tiles the planner formed                         */
+    int32_t max_spread;      /* largest per-channel delay spread inside a DM tile   */
+    int32_t mode;            /* inner-loop mode chosen for nbits                    */
+} spp_sweep_info;
+
+/* Plan on the host from a host delay table int32 [ndm][nchans] (all delays >= 0). */
+int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int64_t nchans,
+                          int64_t nsamps, int nbits, int big, spp_sweep_plan** plan);
+int spp_sweep_plan_destroy(spp_sweep_plan* plan);
+int spp_sweep_plan_info(const spp_sweep_plan* plan, spp_sweep_info* info);
+/* Row lengths (int64 [ndm]) into host memory. */
+int spp_sweep_plan_row_lengths(const spp_sweep_plan* plan, int64_t* out_host);
+/* Copy the plan's device tables into `workspace` (synchronises the stream once). */
+int spp_sweep_plan_upload(spp_sweep_plan* plan, void* workspace, int64_t workspace_bytes,
+                          void* stream);
+/* Run the sweep on a packed time-major stream [nsamps][nchans] (nbits, bit order of the
+ * plan) into plane [ndm][ld] (ld >= max_len). `workspace` must be the buffer the plan
+ * was uploaded to. spp_sweep_execute = spp_sweep_build (stage 1: the channel-major
+ * interleaved copy of the stream, kept in the workspace) + spp_sweep_run (stage 2: the
+ * DM-tiled sums); the stages are exposed so a caller can time or reuse them. */
+int spp_sweep_execute(const spp_sweep_plan* plan, const uint8_t* packed, float* plane,
+                      int64_t ld, void* workspace, int64_t workspace_bytes, void* stream);
+int spp_sweep_build(const spp_sweep_plan* plan, const uint8_t* packed, void* workspace,
+                    int64_t workspace_bytes, void* stream);
+int spp_sweep_run(const spp_sweep_plan* plan, float* plane, int64_t ld, void* workspace,
+                  int64_t workspace_bytes, void* stream);
+
+/*
+ * Running normalisation of DM series (no reference counterpart: the running mean/std
+ * pre-step before MatchedFilter). For each row i of length row_len_host[i]:
+ *   z[t] = (x[t] - m[t]) / s[t], m/s = mean/std over x padded symmetrically as
+ *   stats.running_filter pads (core/stats.py:231-236), `window` samples, float64;
+ *   s == 0 is replaced by 1 (as stats.estimate_zscore does, core/stats.py:469-471).
+ * z must not alias x. workspace: spp_running_normalise_workspace_size bytes.
+ * Synchronises the stream before returning.
+ */
+int spp_running_normalise_workspace_size(int64_t nrows, const int64_t* row_len_host,
+                                         int32_t window, int64_t* bytes);
+int spp_running_normalise(const float* x, int64_t ldx, int64_t nrows,
+                          const int64_t* row_len_host, int32_t window, float* z, int64_t ldz,
+                          void* workspace, int64_t workspace_bytes, void* stream);
+
+/*
+ * Boxcar matched filter with argmax per row. Replaces MatchedFilter._compute
+ * (core/filters.py:325-338) with temp_kind "boxcar" on zscores that are already
+ * normalised (loc/scale "norm"), i.e. kernels.convolve_templates
+ * (core/kernels.py:861-906) in closed form:
+ *   conv_w[t] = (sum_{j<w} zt[(t+j) mod Ng] - w*S/Ng) / sqrt(w*(1 - w/Ng)),
+ *   zt = circular pad of z to Ng = 5-smooth good size >= n (kernels.py:998-1005),
+ *   S = sum(zt); then the first maximum over [width][t] in template-major order
+ *   (np.argmax over convs, filters.py:333-338).
+ * widths_host: int32 [nwidths] (1 <= w <= 1024, nwidths <= 32). Outputs per row (device):
+ * snr (float32 conv value at the maximum), peak (t), widx (index into widths).
+ * workspace: spp_boxcar_mf_workspace_size bytes. Synchronises the stream.
+ */
+int spp_boxcar_mf_workspace_size(int64_t nrows, int64_t max_len, int64_t* bytes);
+int spp_boxcar_mf(const float* z, int64_t ldz, int64_t nrows, const int64_t* row_len_host,
+                  const int32_t* widths_host, int32_t nwidths, float* snr, int32_t* peak,
+                  int32_t* widx, void* workspace, int64_t workspace_bytes, void* stream);
+
+/*
+ * Template bank convolution. Replaces kernels.convolve_templates (core/kernels.py:861-906)
+ * for any template: convs[k][t] = (sum_m h_k[m] zt[(t - ref_k + m) mod Ng] - mu_k*S) /
+ * sigma_k with mu_k = sum(h_k)/Ng and sigma_k = sqrt(sum(h_k^2) - sum(h_k)^2/Ng) (the
+ * normalize_template of core/kernels.py:1008-1027 applied to the padded template;
+ * sigma_k == 0 leaves the numerator unscaled). Computed in float64, stored float32.
+ * temps_host: concatenated float32 template samples; temp_len_host / ref_bin_host: int32
+ * [ntemps]. convs: float32 [ntemps][n]. workspace: spp_convolve_templates_workspace_size.
+ */
+int spp_convolve_templates_workspace_size(int64_t n, int32_t ntemps, int64_t total_taps,
+                                          int64_t* bytes);
+int spp_convolve_templates(const float* data, int64_t n, const float* temps_host,
+                           const int32_t* temp_len_host, const int32_t* ref_bin_host,
+                           int32_t ntemps, float* convs, void* workspace,
+                           int64_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPP_HIP_H */

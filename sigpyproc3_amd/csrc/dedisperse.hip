@@ -1,0 +1,336 @@
+// One-DM dedispersion and subbanding of a gulp, with the sub-byte unpack fused into
+// the LDS staging (the "one DM per launch" form).
+//
+// Gulp layout: file order, [nsamps][nchans] packed nbits (1/2/4/8, bit order `big`)
+// or float32. A block owns DD_T consecutive output samples and a group of channels.
+// For each stage it picks the longest run of channels (at most DD_CC) whose delay
+// spread fits the LDS rows, stages rows [t0 + dmin, t0 + DD_T + dmax) of those
+// channels as one byte per sample (row stride 68 B = 17 dwords, so the per-sample
+// column reads below are bank-conflict free), and every thread adds its two outputs'
+// samples. Integer sums are exact (every partial sum < 2^24), so splitting channels
+// across blocks and combining with float atomics is bit-exact and order-free.
+//
+// Work per output sample: nchans adds; HBM bytes per gulp: nsamps * nchans * nbits/8
+// read + 4 * nout written. One DM per launch is HBM-bound.
+#include "spp_common.hpp"
+
+#include <algorithm>
+#include <type_traits>
+
+namespace {
+
+constexpr int DD_THREADS = 256;
+constexpr int DD_T = 512;       // output samples per block (2 per thread)
+constexpr int DD_CC = 64;       // channels per stage at most
+constexpr int DD_ROWS = 768;    // staged rows per stage: spread <= DD_ROWS - DD_T
+constexpr int DD_STRIDE = 68;   // bytes per staged row
+constexpr int DDF_CC = 16;      // float32 path: channels per stage
+constexpr int DDF_ROWS = 768;
+constexpr int DDF_STRIDE = 17;  // floats per staged row
+
+template <int NBITS, bool BIG>
+__device__ __forceinline__ uint32_t packed_sample(const uint8_t* __restrict__ in, int64_t row,
+                                                  int64_t row_bytes, int64_t c) {
+    if constexpr (NBITS == 8) {
+        return in[row * row_bytes + c];
+    } else {
+        constexpr int PER = 8 / NBITS;
+        const uint32_t b = in[row * row_bytes + c / PER];
+        const int k = static_cast<int>(c % PER);
+        const int shift = BIG ? (8 - (k + 1) * NBITS) : k * NBITS;
+        return (b >> shift) & ((1u << NBITS) - 1u);
+    }
+}
+
+// Greedy stage: the longest prefix of sdel[0..ncand) whose spread <= max_spread.
+__device__ __forceinline__ int stage_extent(const int32_t* sdel, int ncand, int max_spread,
+                                            int& dmin, int& dmax) {
+    dmin = sdel[0];
+    dmax = sdel[0];
+    int n = 1;
+    while (n < ncand) {
+        const int d = sdel[n];
+        const int lo = min(dmin, d), hi = max(dmax, d);
+        if (hi - lo > max_spread) break;
+        dmin = lo;
+        dmax = hi;
+        ++n;
+    }
+    return n;
+}
+
+// MODE 0: dedisperse (out[out_start + t] = sum). MODE 1: subband (out[t*nsubs + sub] += run sum).
+template <int NBITS, bool BIG, int MODE>
+__global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
+    const uint8_t* __restrict__ in, const int32_t* __restrict__ delays,
+    const int32_t* __restrict__ chan_to_sub, float* __restrict__ out, int64_t nchans, int64_t nsamps,
+    int64_t nout, int64_t out_start, int64_t row_bytes, int64_t chans_per_group, int use_atomic,
+    int64_t nsubs, int64_t out_size) {
+    __shared__ uint8_t tile[DD_ROWS * DD_STRIDE];
+    __shared__ int32_t sdel[DD_CC];
+    __shared__ int32_t ssub[DD_CC];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int64_t t0 = static_cast<int64_t>(blockIdx.x) * DD_T;
+    const int64_t cb = static_cast<int64_t>(blockIdx.y) * chans_per_group;
+    const int64_t ce = min(nchans, cb + chans_per_group);
+    const int64_t ta = t0 + tid;
+    const int64_t tb = t0 + DD_THREADS + tid;
+    uint32_t acc0 = 0, acc1 = 0;
+    int cur_sub = -1;
+
+    auto flush_sub = [&](int sub) {
+        if (sub >= 0) {
+            const int64_t ia = ta * nsubs + sub, ib = tb * nsubs + sub;
+            if (ta < nout && ia < out_size) atomicAdd(out + ia, static_cast<float>(acc0));
+            if (tb < nout && ib < out_size) atomicAdd(out + ib, static_cast<float>(acc1));
+        }
+        acc0 = 0;
+        acc1 = 0;
+    };
+
+    for (int64_t c = cb; c < ce;) {
+        const int ncand = static_cast<int>(min<int64_t>(DD_CC, ce - c));
+        __syncthreads();
+        if (tid < ncand) {
+            sdel[tid] = delays[c + tid];
+            if constexpr (MODE == 1) ssub[tid] = chan_to_sub[c + tid];
+        }
+        __syncthreads();
+        int dmin, dmax;
+        const int n = stage_extent(sdel, ncand, DD_ROWS - DD_T, dmin, dmax);
+        const int nrows = DD_T + (dmax - dmin);
+        const int64_t rbase = t0 + dmin;
+        if (lane < n) {
+            for (int r = wave; r < nrows; r += DD_THREADS / 64) {
+                int64_t row = rbase + r;
+                if (row < 0) row += nsamps;  // numba wraparound of a negative index
+                uint32_t v = 0;
+                if (row >= 0 && row < nsamps) v = packed_sample<NBITS, BIG>(in, row, row_bytes, c + lane);
+                tile[r * DD_STRIDE + lane] = static_cast<uint8_t>(v);
+            }
+        }
+        __syncthreads();
+        for (int k = 0; k < n; ++k) {
+            if constexpr (MODE == 1) {
+                const int sub = ssub[k];
+                if (sub != cur_sub) {
+                    flush_sub(cur_sub);
+                    cur_sub = sub;
+                }
+            }
+            const int rel = sdel[k] - dmin;
+            acc0 += tile[(tid + rel) * DD_STRIDE + k];
+            acc1 += tile[(tid + DD_THREADS + rel) * DD_STRIDE + k];
+        }
+        c += n;
+    }
+    if constexpr (MODE == 1) {
+        flush_sub(cur_sub);
+    } else {
+        float* o = out + out_start;
+        if (use_atomic) {
+            if (ta < nout) atomicAdd(o + ta, static_cast<float>(acc0));
+            if (tb < nout) atomicAdd(o + tb, static_cast<float>(acc1));
+        } else {
+            if (ta < nout) o[ta] = static_cast<float>(acc0);
+            if (tb < nout) o[tb] = static_cast<float>(acc1);
+        }
+    }
+}
+
+// float32 samples, channels summed in order: float64 for dedisperse (kernels.py:357-361
+// accumulates in float64), float32 for subband (kernels.py:397-401 adds into the float32
+// output); MODE as above.
+template <int MODE>
+__global__ __launch_bounds__(DD_THREADS) void k_gulp_f32(
+    const float* __restrict__ in, const int32_t* __restrict__ delays,
+    const int32_t* __restrict__ chan_to_sub, float* __restrict__ out, int64_t nchans, int64_t nsamps,
+    int64_t nout, int64_t out_start, int64_t nsubs, int64_t out_size) {
+    __shared__ float tile[DDF_ROWS * DDF_STRIDE];
+    __shared__ int32_t sdel[DDF_CC];
+    __shared__ int32_t ssub[DDF_CC];
+    const int tid = threadIdx.x;
+    const int sub_lane = tid & 15;
+    const int row_grp = tid >> 4;
+    const int64_t t0 = static_cast<int64_t>(blockIdx.x) * DD_T;
+    const int64_t ta = t0 + tid;
+    const int64_t tb = t0 + DD_THREADS + tid;
+    using acc_t = std::conditional_t<MODE == 0, double, float>;
+    acc_t acc0 = 0, acc1 = 0;
+    int cur_sub = -1;
+
+    auto flush_sub = [&](int sub) {
+        if (sub >= 0) {
+            const int64_t ia = ta * nsubs + sub, ib = tb * nsubs + sub;
+            if (ta < nout && ia < out_size) atomicAdd(out + ia, static_cast<float>(acc0));
+            if (tb < nout && ib < out_size) atomicAdd(out + ib, static_cast<float>(acc1));
+        }
+        acc0 = 0;
+        acc1 = 0;
+    };
+
+    for (int64_t c = 0; c < nchans;) {
+        const int ncand = static_cast<int>(min<int64_t>(DDF_CC, nchans - c));
+        __syncthreads();
+        if (tid < ncand) {
+            sdel[tid] = delays[c + tid];
+            if constexpr (MODE == 1) ssub[tid] = chan_to_sub[c + tid];
+        }
+        __syncthreads();
+        int dmin, dmax;
+        const int n = stage_extent(sdel, ncand, DDF_ROWS - DD_T, dmin, dmax);
+        const int nrows = DD_T + (dmax - dmin);
+        const int64_t rbase = t0 + dmin;
+        if (sub_lane < n) {
+            for (int r = row_grp; r < nrows; r += DD_THREADS / 16) {
+                int64_t row = rbase + r;
+                if (row < 0) row += nsamps;
+                float v = 0.f;
+                if (row >= 0 && row < nsamps) v = in[row * nchans + c + sub_lane];
+                tile[r * DDF_STRIDE + sub_lane] = v;
+            }
+        }
+        __syncthreads();
+        for (int k = 0; k < n; ++k) {
+            if constexpr (MODE == 1) {
+                const int sub = ssub[k];
+                if (sub != cur_sub) {
+                    flush_sub(cur_sub);
+                    cur_sub = sub;
+                }
+            }
+            const int rel = sdel[k] - dmin;
+            acc0 += static_cast<acc_t>(tile[(tid + rel) * DDF_STRIDE + k]);
+            acc1 += static_cast<acc_t>(tile[(tid + DD_THREADS + rel) * DDF_STRIDE + k]);
+        }
+        c += n;
+    }
+    if constexpr (MODE == 1) {
+        flush_sub(cur_sub);
+    } else {
+        if (ta < nout) out[out_start + ta] = static_cast<float>(acc0);
+        if (tb < nout) out[out_start + tb] = static_cast<float>(acc1);
+    }
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, int64_t n, float v) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = v;
+}
+
+int fill_f32(float* p, int64_t n, hipStream_t st) {
+    if (n <= 0) return SPP_OK;
+    const int64_t blocks = std::min<int64_t>(spp::ceil_div(n, 256), int64_t(spp::device_cu_count()) * 8);
+    hipLaunchKernelGGL(k_fill_f32, dim3(blocks), dim3(256), 0, st, p, n, 0.f);
+    SPP_LAUNCHED("k_fill_f32");
+    return SPP_OK;
+}
+
+template <int MODE>
+int launch_int(const uint8_t* in, int nbits, int big, const int32_t* delays, const int32_t* cts,
+               float* out, int64_t nchans, int64_t nsamps, int64_t nout, int64_t out_start,
+               int64_t nsubs, int64_t out_size, hipStream_t st) {
+    const int64_t row_bytes = nchans * nbits / 8;
+    const int64_t bx = spp::ceil_div(nout, DD_T);
+    SPP_REQUIRE(bx <= 0x7fffffffLL, "gulp too long");
+    // Split channels across blocks until the grid fills the device (integer sums are
+    // exact, so the atomic combination is order-free).
+    const int64_t cus = spp::device_cu_count();
+    int64_t groups = 1;
+    const int64_t maxval = (nbits == 8) ? 255 : ((1 << nbits) - 1);
+    if (nchans * maxval < (int64_t(1) << 24)) {
+        groups = std::max<int64_t>(1, std::min<int64_t>(spp::ceil_div(4 * cus, bx),
+                                                        spp::ceil_div(nchans, DD_CC)));
+    }
+    int64_t cpg = spp::ceil_div(nchans, groups);
+    cpg = spp::ceil_div(cpg, 8) * 8;
+    groups = spp::ceil_div(nchans, cpg);
+    const int use_atomic = (MODE == 0 && groups > 1) ? 1 : 0;
+    if (use_atomic) {
+        int rc = fill_f32(out + out_start, nout, st);
+        if (rc) return rc;
+    }
+    dim3 grid(static_cast<unsigned>(bx), static_cast<unsigned>(groups));
+#define SPP_GULP_LAUNCH(NB, BG)                                                               \
+    hipLaunchKernelGGL((k_gulp_int<NB, BG, MODE>), grid, dim3(DD_THREADS), 0, st, in, delays, cts, \
+                       out, nchans, nsamps, nout, out_start, row_bytes, cpg, use_atomic, nsubs,   \
+                       out_size)
+    switch (nbits * 2 + (big ? 1 : 0)) {
+        case 2: SPP_GULP_LAUNCH(1, false); break;
+        case 3: SPP_GULP_LAUNCH(1, true); break;
+        case 4: SPP_GULP_LAUNCH(2, false); break;
+        case 5: SPP_GULP_LAUNCH(2, true); break;
+        case 8: SPP_GULP_LAUNCH(4, false); break;
+        case 9: SPP_GULP_LAUNCH(4, true); break;
+        default: SPP_GULP_LAUNCH(8, false); break;
+    }
+#undef SPP_GULP_LAUNCH
+    SPP_LAUNCHED("k_gulp_int");
+    return SPP_OK;
+}
+
+int check_common(const void* in, int nbits, int64_t nchans, int64_t nsamps, const int32_t* delays) {
+    SPP_REQUIRE(nbits == 1 || nbits == 2 || nbits == 4 || nbits == 8 || nbits == 32,
+                "nbits must be 1, 2, 4, 8 or 32, got %d", nbits);
+    SPP_REQUIRE(nchans > 0, "nchans must be > 0");
+    SPP_REQUIRE(nsamps > 0, "nsamps must be > 0");
+    SPP_REQUIRE((nchans * nbits) % 8 == 0, "nchans * nbits must be a multiple of 8");
+    SPP_REQUIRE(in != nullptr && delays != nullptr, "null buffer");
+    return SPP_OK;
+}
+
+}  // namespace
+
+extern "C" int spp_dedisperse(const void* in, int nbits, int big, int64_t nchans, int64_t nsamps,
+                              const int32_t* delays, int64_t max_delay, float* out,
+                              int64_t out_start, void* stream) {
+    spp::clear_error();
+    int rc = check_common(in, nbits, nchans, nsamps, delays);
+    if (rc) return rc;
+    SPP_REQUIRE(out != nullptr, "null buffer");
+    SPP_REQUIRE(out_start >= 0, "out_start must be >= 0");
+    SPP_REQUIRE(max_delay >= 0, "max_delay must be >= 0");
+    hipStream_t st = spp::as_stream(stream);
+    const int64_t nout = nsamps - max_delay;
+    if (nout <= 0) return SPP_OK;  // the reference loop body never runs
+    if (nbits == 32) {
+        const int64_t bx = spp::ceil_div(nout, DD_T);
+        hipLaunchKernelGGL((k_gulp_f32<0>), dim3(static_cast<unsigned>(bx)), dim3(DD_THREADS), 0, st,
+                           static_cast<const float*>(in), delays, nullptr, out, nchans, nsamps, nout,
+                           out_start, int64_t(0), int64_t(0));
+        SPP_LAUNCHED("k_gulp_f32");
+        return SPP_OK;
+    }
+    return launch_int<0>(static_cast<const uint8_t*>(in), nbits, big, delays, nullptr, out, nchans,
+                         nsamps, nout, out_start, 0, 0, st);
+}
+
+extern "C" int spp_subband(const void* in, int nbits, int big, int64_t nchans, int64_t nsamps,
+                           const int32_t* delays, const int32_t* chan_to_sub, int64_t maxdelay,
+                           int64_t nsubs, float* out, int64_t out_size, void* stream) {
+    spp::clear_error();
+    int rc = check_common(in, nbits, nchans, nsamps, delays);
+    if (rc) return rc;
+    SPP_REQUIRE(out != nullptr && chan_to_sub != nullptr, "null buffer");
+    SPP_REQUIRE(nsubs > 0, "nsubs must be > 0");
+    SPP_REQUIRE(out_size >= 0, "out_size must be >= 0");
+    SPP_REQUIRE(maxdelay >= 0, "maxdelay must be >= 0");
+    hipStream_t st = spp::as_stream(stream);
+    const int64_t nout = nsamps - maxdelay;
+    if (nout <= 0) return SPP_OK;
+    rc = fill_f32(out, std::min<int64_t>(out_size, nout * nsubs), st);
+    if (rc) return rc;
+    if (nbits == 32) {
+        const int64_t bx = spp::ceil_div(nout, DD_T);
+        hipLaunchKernelGGL((k_gulp_f32<1>), dim3(static_cast<unsigned>(bx)), dim3(DD_THREADS), 0, st,
+                           static_cast<const float*>(in), delays, chan_to_sub, out, nchans, nsamps,
+                           nout, int64_t(0), nsubs, out_size);
+        SPP_LAUNCHED("k_gulp_f32");
+        return SPP_OK;
+    }
+    return launch_int<1>(static_cast<const uint8_t*>(in), nbits, big, delays, chan_to_sub, out,
+                         nchans, nsamps, nout, 0, nsubs, out_size, st);
+}

@@ -1,0 +1,617 @@
+// Matched-filter stage after the sweep: running normalisation, boxcar matched filter
+// with a per-row argmax, and the general template-bank convolution.
+//
+// Everything here is a small fraction of the sweep's cost (a few HBM passes over the
+// plane). Reductions use fixed orders, so results are deterministic run to run.
+//
+// Running normalisation (one DM series x of length n, window W):
+//   p = x padded symmetrically by (W/2, W/2 - (W even)) samples (the padding of
+//   core/stats.py:231-236), d = p - x[0], P(k) = sum_{i<k} d[i] (float64, likewise P2
+//   for d^2). For output t: S1 = P(t+W) - P(t), S2 = P2(t+W) - P2(t),
+//   m = S1/W, v = max(S2/W - m*m, 0), s = sqrt(v) (s == 0 -> 1),
+//   z[t] = float32((x[t] - (m + x[0])) / s).
+// P is formed by segment totals (k_seg_totals), a per-row exclusive scan of the totals
+// (k_seg_scan), and a per-block local scan (k_norm), so each output reads O(1) data.
+//
+// Boxcar MF (core/kernels.py:861-906 closed form, row length n, Ng = good size >= n):
+//   zt[i] = z[(i mod Ng) mod n], S = sum_{i<Ng} zt[i],
+//   conv_w[t] = (sum_{j<w} zt[t+j] - w*S/Ng) / sqrt(w*(1 - w/Ng)),  t < n,
+//   argmax over [width][t] in template-major order with the first maximum kept.
+#include "spp_common.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+namespace {
+
+constexpr int RN_SEG = 2048;     // padded samples per segment / outputs per k_norm block
+constexpr int RN_THREADS = 256;
+constexpr int RN_PER = RN_SEG / RN_THREADS;  // 8
+constexpr int MF_B = 2048;       // outputs per MF block
+constexpr int MF_THREADS = 256;
+constexpr int MF_WMAX = 1024;    // largest boxcar width supported
+constexpr int MF_MAXW = 32;      // largest number of widths
+constexpr int CT_B = 1024;       // outputs per template-convolution block
+constexpr int CT_TAPS = 2048;    // largest template length
+
+int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+__device__ __forceinline__ int64_t sym_index(int64_t i, int64_t n) {
+    // numpy 'symmetric' extension: period 2n, mirrored including the edge sample.
+    const int64_t two_n = 2 * n;
+    int64_t m = i % two_n;
+    if (m < 0) m += two_n;
+    return (m < n) ? m : (two_n - 1 - m);
+}
+
+// Block-wide sum of a double (fixed tree order). All threads get the result.
+__device__ double block_sum(double v, double* red) {
+    const int tid = threadIdx.x;
+    red[tid] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// Exclusive block scan of two doubles per thread (Hillis-Steele on LDS, fixed order).
+__device__ void block_exscan2(double& a, double& b, double* sa, double* sb) {
+    const int tid = threadIdx.x;
+    sa[tid] = a;
+    sb[tid] = b;
+    __syncthreads();
+    for (int off = 1; off < static_cast<int>(blockDim.x); off <<= 1) {
+        double ta = 0.0, tb = 0.0;
+        if (tid >= off) {
+            ta = sa[tid - off];
+            tb = sb[tid - off];
+        }
+        __syncthreads();
+        if (tid >= off) {
+            sa[tid] += ta;
+            sb[tid] += tb;
+        }
+        __syncthreads();
+    }
+    const double ia = sa[tid] - a, ib = sb[tid] - b;  // inclusive - own = exclusive
+    __syncthreads();
+    a = ia;
+    b = ib;
+}
+
+// ---------------------------------------------------------------- running normalise
+
+struct RowDesc {
+    int64_t n;     // row length
+    int64_t nseg;  // segments of the padded row
+};
+
+// Segment totals of d = p - x[0] and d^2 over padded indices [s*SEG, (s+1)*SEG).
+__global__ __launch_bounds__(RN_THREADS) void k_seg_totals(const float* __restrict__ x, int64_t ldx,
+                                                           const RowDesc* __restrict__ rows,
+                                                           int32_t window, int64_t seg_pitch,
+                                                           double* __restrict__ tot) {
+    __shared__ double red[RN_THREADS];
+    const int64_t row = blockIdx.y;
+    const RowDesc rd = rows[row];
+    const int64_t s = blockIdx.x;
+    if (s >= rd.nseg) return;
+    const float* __restrict__ xr = x + row * ldx;
+    const int64_t L = window / 2;
+    const int64_t np = rd.n + window - 1;
+    const double ref = xr[0];
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = threadIdx.x; i < RN_SEG; i += RN_THREADS) {
+        const int64_t k = s * RN_SEG + i;
+        if (k < np) {
+            const double d = static_cast<double>(xr[sym_index(k - L, rd.n)]) - ref;
+            s1 += d;
+            s2 += d * d;
+        }
+    }
+    s1 = block_sum(s1, red);
+    s2 = block_sum(s2, red);
+    if (threadIdx.x == 0) {
+        tot[(row * seg_pitch + s) * 2 + 0] = s1;
+        tot[(row * seg_pitch + s) * 2 + 1] = s2;
+    }
+}
+
+// Exclusive scan of segment totals per row (sequential per row: a few hundred adds).
+__global__ void k_seg_scan(const RowDesc* __restrict__ rows, int64_t nrows, int64_t seg_pitch,
+                           const double* __restrict__ tot, double* __restrict__ pre) {
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (row >= nrows) return;
+    const int64_t nseg = rows[row].nseg;
+    double a = 0.0, b = 0.0;
+    for (int64_t s = 0; s <= nseg; ++s) {
+        pre[(row * (seg_pitch + 1) + s) * 2 + 0] = a;
+        pre[(row * (seg_pitch + 1) + s) * 2 + 1] = b;
+        if (s < nseg) {
+            a += tot[(row * seg_pitch + s) * 2 + 0];
+            b += tot[(row * seg_pitch + s) * 2 + 1];
+        }
+    }
+}
+
+// z for outputs [c*SEG, c*SEG + SEG) of one row.
+__global__ __launch_bounds__(RN_THREADS) void k_norm(const float* __restrict__ x, int64_t ldx,
+                                                     const RowDesc* __restrict__ rows, int32_t window,
+                                                     int64_t seg_pitch, const double* __restrict__ pre,
+                                                     float* __restrict__ z, int64_t ldz) {
+    __shared__ double sa[RN_THREADS], sb[RN_THREADS], red[RN_THREADS];
+    const int64_t row = blockIdx.y;
+    const RowDesc rd = rows[row];
+    const int64_t a0 = static_cast<int64_t>(blockIdx.x) * RN_SEG;
+    if (a0 >= rd.n) return;
+    const int tid = threadIdx.x;
+    const float* __restrict__ xr = x + row * ldx;
+    const int64_t n = rd.n;
+    const int64_t L = window / 2;
+    const double ref = xr[0];
+    const double* __restrict__ pr = pre + row * (seg_pitch + 1) * 2;
+
+    // Range A: padded indices a0 .. a0 + SEG (segment aligned). Range B: a0 + W ...
+    const int64_t b0 = a0 + window;
+    const int64_t sbi = b0 / RN_SEG;
+    const int64_t boff = b0 - sbi * RN_SEG;
+    // P(b0) = pre[sbi] + sum of d over [sbi*SEG, b0).
+    double pb1 = 0.0, pb2 = 0.0;
+    for (int64_t i = tid; i < boff; i += RN_THREADS) {
+        const double d = static_cast<double>(xr[sym_index(sbi * RN_SEG + i - L, n)]) - ref;
+        pb1 += d;
+        pb2 += d * d;
+    }
+    pb1 = block_sum(pb1, red);
+    pb2 = block_sum(pb2, red);
+    pb1 += pr[sbi * 2 + 0];
+    pb2 += pr[sbi * 2 + 1];
+    const int64_t sai = a0 / RN_SEG;
+    const double pa1 = pr[sai * 2 + 0], pa2 = pr[sai * 2 + 1];
+
+    // Each thread owns outputs a0 + RN_PER*tid + j (j < RN_PER).
+    double da1[RN_PER], da2[RN_PER], db1[RN_PER], db2[RN_PER];
+    double ta1 = 0.0, ta2 = 0.0, tb1 = 0.0, tb2 = 0.0;
+    const int64_t np = n + window - 1;
+#pragma unroll
+    for (int j = 0; j < RN_PER; ++j) {
+        const int64_t ka = a0 + RN_PER * tid + j;
+        const int64_t kb = b0 + RN_PER * tid + j;
+        double d = 0.0;
+        if (ka < np) d = static_cast<double>(xr[sym_index(ka - L, n)]) - ref;
+        da1[j] = ta1;  // prefix before element j
+        da2[j] = ta2;
+        ta1 += d;
+        ta2 += d * d;
+        double e = 0.0;
+        if (kb < np) e = static_cast<double>(xr[sym_index(kb - L, n)]) - ref;
+        db1[j] = tb1;
+        db2[j] = tb2;
+        tb1 += e;
+        tb2 += e * e;
+    }
+    double ea = ta1, eb = ta2;
+    block_exscan2(ea, eb, sa, sb);
+    double fa = tb1, fb = tb2;
+    block_exscan2(fa, fb, sa, sb);
+    float* __restrict__ zr = z + row * ldz;
+    const double invw = 1.0 / static_cast<double>(window);
+#pragma unroll
+    for (int j = 0; j < RN_PER; ++j) {
+        const int64_t t = a0 + RN_PER * tid + j;
+        if (t < n) {
+            const double s1 = (pb1 + fa + db1[j]) - (pa1 + ea + da1[j]);
+            const double s2 = (pb2 + fb + db2[j]) - (pa2 + eb + da2[j]);
+            const double m = s1 * invw;
+            double v = s2 * invw - m * m;
+            v = v > 0.0 ? v : 0.0;
+            double sd = sqrt(v);
+            if (sd == 0.0) sd = 1.0;
+            zr[t] = static_cast<float>((static_cast<double>(xr[t]) - (m + ref)) / sd);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- boxcar MF
+
+struct MfRow {
+    int64_t n;
+    int64_t ng;
+};
+
+struct Best {
+    float val;
+    int32_t w;
+    int64_t t;
+};
+
+__device__ __forceinline__ bool better(float v, int32_t w, int64_t t, const Best& b) {
+    if (v > b.val) return true;
+    if (v < b.val || v != v) return false;
+    return (w < b.w) || (w == b.w && t < b.t);
+}
+
+// S = sum_{i<Ng} zt[i] per row.
+__global__ __launch_bounds__(MF_THREADS) void k_circ_sum(const float* __restrict__ z, int64_t ldz,
+                                                         const MfRow* __restrict__ rows,
+                                                         double* __restrict__ S) {
+    __shared__ double red[MF_THREADS];
+    const int64_t row = blockIdx.x;
+    const MfRow r = rows[row];
+    const float* __restrict__ zr = z + row * ldz;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < r.ng; i += MF_THREADS) s += static_cast<double>(zr[i % r.n]);
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) S[row] = s;
+}
+
+__global__ __launch_bounds__(MF_THREADS) void k_boxcar_part(
+    const float* __restrict__ z, int64_t ldz, const MfRow* __restrict__ rows,
+    const double* __restrict__ S, const int32_t* __restrict__ widths, int nw, int wmax,
+    int64_t chunk_pitch, Best* __restrict__ part) {
+    __shared__ double Q[MF_B + MF_WMAX + 1];
+    __shared__ double sa[MF_THREADS], sb[MF_THREADS];
+    __shared__ double coef_mean[MF_MAXW], coef_inv[MF_MAXW];
+    __shared__ Best bred[MF_THREADS];
+    const int64_t row = blockIdx.y;
+    const MfRow r = rows[row];
+    const int64_t a = static_cast<int64_t>(blockIdx.x) * MF_B;
+    const int tid = threadIdx.x;
+    Best best;
+    best.val = -FLT_MAX;
+    best.w = 0x7fffffff;
+    best.t = INT64_MAX;
+    if (a >= r.n) {
+        if (tid == 0) part[row * chunk_pitch + blockIdx.x] = best;
+        return;
+    }
+    const float* __restrict__ zr = z + row * ldz;
+    const double srow = S[row];
+    const double ng = static_cast<double>(r.ng);
+    if (tid < nw) {
+        const double w = widths[tid];
+        coef_mean[tid] = w * srow / ng;
+        coef_inv[tid] = 1.0 / sqrt(w * (1.0 - w / ng));
+    }
+    const int64_t lw = min<int64_t>(MF_B, r.n - a);
+    const int nel = static_cast<int>(lw) + wmax;  // zt[a .. a + nel)
+    constexpr int KE = (MF_B + MF_WMAX + MF_THREADS - 1) / MF_THREADS;
+    double loc[KE];
+    double tot = 0.0;
+#pragma unroll
+    for (int e = 0; e < KE; ++e) {
+        const int k = tid * KE + e;
+        double v = 0.0;
+        if (k < nel) {
+            int64_t i = (a + k) % r.ng;
+            v = static_cast<double>(zr[i % r.n]);
+        }
+        tot += v;
+        loc[e] = tot;
+    }
+    double ex = tot, dummy = 0.0;
+    block_exscan2(ex, dummy, sa, sb);
+    if (tid == 0) Q[0] = 0.0;
+#pragma unroll
+    for (int e = 0; e < KE; ++e) {
+        const int k = tid * KE + e;
+        if (k < nel) Q[k + 1] = ex + loc[e];
+    }
+    __syncthreads();
+    for (int u = tid; u < lw; u += MF_THREADS) {
+        for (int wi = 0; wi < nw; ++wi) {
+            const int w = widths[wi];
+            const double conv = (Q[u + w] - Q[u] - coef_mean[wi]) * coef_inv[wi];
+            const float cv = static_cast<float>(conv);
+            if (better(cv, wi, a + u, best)) {
+                best.val = cv;
+                best.w = wi;
+                best.t = a + u;
+            }
+        }
+    }
+    bred[tid] = best;
+    __syncthreads();
+    for (int s = MF_THREADS / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            const Best o = bred[tid + s];
+            if (better(o.val, o.w, o.t, bred[tid])) bred[tid] = o;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) part[row * chunk_pitch + blockIdx.x] = bred[0];
+}
+
+__global__ void k_boxcar_final(const Best* __restrict__ part, int64_t nrows, int64_t chunk_pitch,
+                               const MfRow* __restrict__ rows, float* __restrict__ snr,
+                               int32_t* __restrict__ peak, int32_t* __restrict__ widx) {
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (row >= nrows) return;
+    const int64_t nch = (rows[row].n + MF_B - 1) / MF_B;
+    Best b = part[row * chunk_pitch];
+    for (int64_t c = 1; c < nch; ++c) {
+        const Best o = part[row * chunk_pitch + c];
+        if (better(o.val, o.w, o.t, b)) b = o;
+    }
+    snr[row] = b.val;
+    peak[row] = static_cast<int32_t>(b.t);
+    widx[row] = b.w;
+}
+
+// ---------------------------------------------------------------- template bank
+
+struct TempDesc {
+    int32_t len;
+    int32_t ref;
+    int64_t off;   // offset of the taps in the tap array
+    double mu;     // sum(h) / Ng
+    double inv;    // 1 / sigma (or 1 if sigma == 0)
+};
+
+__global__ __launch_bounds__(MF_THREADS) void k_template_conv(
+    const float* __restrict__ z, int64_t n, int64_t ng, const double* __restrict__ S,
+    const float* __restrict__ taps, const TempDesc* __restrict__ temps, float* __restrict__ convs) {
+    __shared__ float zs[CT_B + CT_TAPS];
+    __shared__ float hs[CT_TAPS];
+    const int64_t a = static_cast<int64_t>(blockIdx.x) * CT_B;
+    const TempDesc td = temps[blockIdx.y];
+    const int tid = threadIdx.x;
+    // zt[(t - ref + m) mod Ng] for t in [a, a + CT_B), m < len.
+    const int nel = CT_B + td.len - 1;
+    for (int k = tid; k < nel; k += MF_THREADS) {
+        int64_t i = (a - td.ref + k) % ng;
+        if (i < 0) i += ng;
+        zs[k] = z[i % n];
+    }
+    for (int m = tid; m < td.len; m += MF_THREADS) hs[m] = taps[td.off + m];
+    __syncthreads();
+    const double mus = td.mu * S[0];
+    for (int u = tid; u < CT_B; u += MF_THREADS) {
+        const int64_t t = a + u;
+        if (t >= n) break;
+        double acc = 0.0;
+        for (int m = 0; m < td.len; ++m) acc += static_cast<double>(hs[m]) * static_cast<double>(zs[u + m]);
+        convs[static_cast<int64_t>(blockIdx.y) * n + t] = static_cast<float>((acc - mus) * td.inv);
+    }
+}
+
+__global__ __launch_bounds__(MF_THREADS) void k_circ_sum1(const float* __restrict__ z, int64_t n,
+                                                          int64_t ng, double* __restrict__ S) {
+    __shared__ double red[MF_THREADS];
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < ng; i += MF_THREADS) s += static_cast<double>(z[i % n]);
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) S[0] = s;
+}
+
+int64_t good_size_real(int64_t n) {
+    // Smallest 2^a 3^b 5^c >= n (pocketfft's good_size for real transforms).
+    if (n <= 6) return n;
+    int64_t best = 2 * n;
+    for (int64_t f5 = 1; f5 < best; f5 *= 5)
+        for (int64_t f35 = f5; f35 < best; f35 *= 3) {
+            int64_t x = f35;
+            while (x < n) x *= 2;
+            best = std::min(best, x);
+        }
+    return best;
+}
+
+struct NormLayout {
+    int64_t seg_pitch = 0, off_rows = 0, off_tot = 0, off_pre = 0, bytes = 0;
+};
+
+NormLayout norm_layout(int64_t nrows, const int64_t* row_len, int32_t window) {
+    NormLayout l;
+    int64_t maxseg = 1;
+    for (int64_t i = 0; i < nrows; ++i)
+        maxseg = std::max<int64_t>(maxseg, (row_len[i] + window - 1 + RN_SEG - 1) / RN_SEG);
+    l.seg_pitch = maxseg;
+    int64_t off = 0;
+    l.off_rows = off; off = align256(off + static_cast<int64_t>(sizeof(RowDesc)) * nrows);
+    l.off_tot = off; off = align256(off + 16 * nrows * maxseg);
+    l.off_pre = off; off = align256(off + 16 * nrows * (maxseg + 1));
+    l.bytes = off;
+    return l;
+}
+
+}  // namespace
+
+extern "C" int spp_running_normalise_workspace_size(int64_t nrows, const int64_t* row_len_host,
+                                                    int32_t window, int64_t* bytes) {
+    spp::clear_error();
+    SPP_REQUIRE(nrows > 0 && row_len_host != nullptr && bytes != nullptr, "invalid argument");
+    SPP_REQUIRE(window >= 1, "window must be >= 1");
+    *bytes = norm_layout(nrows, row_len_host, window).bytes;
+    return SPP_OK;
+}
+
+extern "C" int spp_running_normalise(const float* x, int64_t ldx, int64_t nrows,
+                                     const int64_t* row_len_host, int32_t window, float* z,
+                                     int64_t ldz, void* workspace, int64_t workspace_bytes,
+                                     void* stream) {
+    spp::clear_error();
+    SPP_REQUIRE(x != nullptr && z != nullptr && workspace != nullptr && row_len_host != nullptr,
+                "null argument");
+    SPP_REQUIRE(nrows > 0, "nrows must be > 0");
+    SPP_REQUIRE(window >= 1, "window must be >= 1");
+    SPP_REQUIRE(static_cast<const void*>(x) != static_cast<const void*>(z), "z must not alias x");
+    std::vector<RowDesc> rows(nrows);
+    int64_t maxseg = 0;
+    for (int64_t i = 0; i < nrows; ++i) {
+        SPP_REQUIRE(row_len_host[i] >= 1 && row_len_host[i] <= ldx && row_len_host[i] <= ldz,
+                    "row %lld: length %lld outside [1, ld]", static_cast<long long>(i),
+                    static_cast<long long>(row_len_host[i]));
+        rows[i].n = row_len_host[i];
+        rows[i].nseg = (row_len_host[i] + window - 1 + RN_SEG - 1) / RN_SEG;
+        maxseg = std::max(maxseg, rows[i].nseg);
+    }
+    const NormLayout l = norm_layout(nrows, row_len_host, window);
+    SPP_REQUIRE(workspace_bytes >= l.bytes, "workspace too small: %lld < %lld",
+                static_cast<long long>(workspace_bytes), static_cast<long long>(l.bytes));
+    SPP_REQUIRE(nrows <= 65535, "nrows must be <= 65535");
+    hipStream_t st = spp::as_stream(stream);
+    char* ws = static_cast<char*>(workspace);
+    RowDesc* drows = reinterpret_cast<RowDesc*>(ws + l.off_rows);
+    double* tot = reinterpret_cast<double*>(ws + l.off_tot);
+    double* pre = reinterpret_cast<double*>(ws + l.off_pre);
+    SPP_HIP(hipMemcpyAsync(drows, rows.data(), sizeof(RowDesc) * nrows, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_seg_totals, dim3(static_cast<unsigned>(maxseg), static_cast<unsigned>(nrows)),
+                       dim3(RN_THREADS), 0, st, x, ldx, drows, window, l.seg_pitch, tot);
+    SPP_LAUNCHED("k_seg_totals");
+    hipLaunchKernelGGL(k_seg_scan, dim3(static_cast<unsigned>(spp::ceil_div(nrows, 64))), dim3(64), 0, st,
+                       drows, nrows, l.seg_pitch, tot, pre);
+    SPP_LAUNCHED("k_seg_scan");
+    int64_t maxn = 0;
+    for (auto& r : rows) maxn = std::max(maxn, r.n);
+    hipLaunchKernelGGL(k_norm,
+                       dim3(static_cast<unsigned>(spp::ceil_div(maxn, RN_SEG)), static_cast<unsigned>(nrows)),
+                       dim3(RN_THREADS), 0, st, x, ldx, drows, window, l.seg_pitch, pre, z, ldz);
+    SPP_LAUNCHED("k_norm");
+    // The row descriptors are read by the kernels above: keep the host copy alive.
+    SPP_HIP(hipStreamSynchronize(st));
+    return SPP_OK;
+}
+
+namespace {
+struct MfLayout {
+    int64_t chunk_pitch = 0, off_rows = 0, off_S = 0, off_w = 0, off_part = 0, bytes = 0;
+};
+MfLayout mf_layout(int64_t nrows, int64_t max_len) {
+    MfLayout l;
+    l.chunk_pitch = std::max<int64_t>(1, (max_len + MF_B - 1) / MF_B);
+    int64_t off = 0;
+    l.off_rows = off; off = align256(off + static_cast<int64_t>(sizeof(MfRow)) * nrows);
+    l.off_S = off; off = align256(off + 8 * nrows);
+    l.off_w = off; off = align256(off + 4 * MF_MAXW);
+    l.off_part = off; off = align256(off + static_cast<int64_t>(sizeof(Best)) * nrows * l.chunk_pitch);
+    l.bytes = off;
+    return l;
+}
+}  // namespace
+
+extern "C" int spp_boxcar_mf_workspace_size(int64_t nrows, int64_t max_len, int64_t* bytes) {
+    spp::clear_error();
+    SPP_REQUIRE(nrows > 0 && max_len > 0 && bytes != nullptr, "invalid argument");
+    *bytes = mf_layout(nrows, max_len).bytes;
+    return SPP_OK;
+}
+
+extern "C" int spp_boxcar_mf(const float* z, int64_t ldz, int64_t nrows, const int64_t* row_len_host,
+                             const int32_t* widths_host, int32_t nwidths, float* snr, int32_t* peak,
+                             int32_t* widx, void* workspace, int64_t workspace_bytes, void* stream) {
+    spp::clear_error();
+    SPP_REQUIRE(z != nullptr && snr != nullptr && peak != nullptr && widx != nullptr &&
+                    workspace != nullptr && widths_host != nullptr && row_len_host != nullptr,
+                "null argument");
+    SPP_REQUIRE(nrows > 0 && nrows <= 65535, "nrows must be in [1, 65535]");
+    SPP_REQUIRE(nwidths >= 1 && nwidths <= MF_MAXW, "nwidths must be in [1, %d]", MF_MAXW);
+    int wmax = 0;
+    for (int i = 0; i < nwidths; ++i) {
+        SPP_REQUIRE(widths_host[i] >= 1 && widths_host[i] <= MF_WMAX, "width %d outside [1, %d]",
+                    widths_host[i], MF_WMAX);
+        wmax = std::max(wmax, widths_host[i]);
+    }
+    std::vector<MfRow> rows(nrows);
+    int64_t maxn = 0;
+    for (int64_t i = 0; i < nrows; ++i) {
+        const int64_t n = row_len_host[i];
+        SPP_REQUIRE(n >= 1 && n <= ldz, "row %lld: length %lld outside [1, ldz]",
+                    static_cast<long long>(i), static_cast<long long>(n));
+        SPP_REQUIRE(wmax <= n, "row %lld: width %d larger than the row (%lld)",
+                    static_cast<long long>(i), wmax, static_cast<long long>(n));
+        rows[i].n = n;
+        rows[i].ng = good_size_real(n);
+        maxn = std::max(maxn, n);
+    }
+    const MfLayout l = mf_layout(nrows, maxn);
+    SPP_REQUIRE(workspace_bytes >= l.bytes, "workspace too small: %lld < %lld",
+                static_cast<long long>(workspace_bytes), static_cast<long long>(l.bytes));
+    hipStream_t st = spp::as_stream(stream);
+    char* ws = static_cast<char*>(workspace);
+    MfRow* drows = reinterpret_cast<MfRow*>(ws + l.off_rows);
+    double* S = reinterpret_cast<double*>(ws + l.off_S);
+    int32_t* dw = reinterpret_cast<int32_t*>(ws + l.off_w);
+    Best* part = reinterpret_cast<Best*>(ws + l.off_part);
+    SPP_HIP(hipMemcpyAsync(drows, rows.data(), sizeof(MfRow) * nrows, hipMemcpyHostToDevice, st));
+    SPP_HIP(hipMemcpyAsync(dw, widths_host, 4 * nwidths, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_circ_sum, dim3(static_cast<unsigned>(nrows)), dim3(MF_THREADS), 0, st, z, ldz,
+                       drows, S);
+    SPP_LAUNCHED("k_circ_sum");
+    hipLaunchKernelGGL(k_boxcar_part,
+                       dim3(static_cast<unsigned>(l.chunk_pitch), static_cast<unsigned>(nrows)),
+                       dim3(MF_THREADS), 0, st, z, ldz, drows, S, dw, nwidths, wmax, l.chunk_pitch, part);
+    SPP_LAUNCHED("k_boxcar_part");
+    hipLaunchKernelGGL(k_boxcar_final, dim3(static_cast<unsigned>(spp::ceil_div(nrows, 64))), dim3(64), 0,
+                       st, part, nrows, l.chunk_pitch, drows, snr, peak, widx);
+    SPP_LAUNCHED("k_boxcar_final");
+    SPP_HIP(hipStreamSynchronize(st));
+    return SPP_OK;
+}
+
+extern "C" int spp_convolve_templates_workspace_size(int64_t n, int32_t ntemps, int64_t total_taps,
+                                                     int64_t* bytes) {
+    spp::clear_error();
+    SPP_REQUIRE(n > 0 && ntemps > 0 && total_taps > 0 && bytes != nullptr, "invalid argument");
+    *bytes = align256(8) + align256(static_cast<int64_t>(sizeof(TempDesc)) * ntemps) +
+             align256(4 * total_taps);
+    return SPP_OK;
+}
+
+extern "C" int spp_convolve_templates(const float* data, int64_t n, const float* temps_host,
+                                      const int32_t* temp_len_host, const int32_t* ref_bin_host,
+                                      int32_t ntemps, float* convs, void* workspace,
+                                      int64_t workspace_bytes, void* stream) {
+    spp::clear_error();
+    SPP_REQUIRE(data != nullptr && convs != nullptr && workspace != nullptr && temps_host != nullptr &&
+                    temp_len_host != nullptr && ref_bin_host != nullptr,
+                "null argument");
+    SPP_REQUIRE(n > 0, "n must be > 0");
+    SPP_REQUIRE(ntemps > 0 && ntemps <= 65535, "ntemps must be in [1, 65535]");
+    const int64_t ng = good_size_real(n);
+    std::vector<TempDesc> td(ntemps);
+    int64_t off = 0;
+    for (int i = 0; i < ntemps; ++i) {
+        const int32_t len = temp_len_host[i];
+        SPP_REQUIRE(len >= 1 && len <= CT_TAPS, "template %d: length %d outside [1, %d]", i, len, CT_TAPS);
+        SPP_REQUIRE(len <= ng, "template %d longer than the padded data", i);
+        SPP_REQUIRE(ref_bin_host[i] >= 0 && ref_bin_host[i] < len, "template %d: ref bin out of range", i);
+        double s = 0.0, s2 = 0.0;
+        for (int m = 0; m < len; ++m) {
+            const double h = temps_host[off + m];
+            s += h;
+            s2 += h * h;
+        }
+        const double var = s2 - s * s / static_cast<double>(ng);
+        td[i].len = len;
+        td[i].ref = ref_bin_host[i];
+        td[i].off = off;
+        td[i].mu = s / static_cast<double>(ng);
+        td[i].inv = var > 0.0 ? 1.0 / std::sqrt(var) : 1.0;
+        off += len;
+    }
+    int64_t need = 0;
+    spp_convolve_templates_workspace_size(n, ntemps, off, &need);
+    SPP_REQUIRE(workspace_bytes >= need, "workspace too small");
+    hipStream_t st = spp::as_stream(stream);
+    char* ws = static_cast<char*>(workspace);
+    double* S = reinterpret_cast<double*>(ws);
+    TempDesc* dtd = reinterpret_cast<TempDesc*>(ws + align256(8));
+    float* taps = reinterpret_cast<float*>(ws + align256(8) + align256(sizeof(TempDesc) * ntemps));
+    SPP_HIP(hipMemcpyAsync(dtd, td.data(), sizeof(TempDesc) * ntemps, hipMemcpyHostToDevice, st));
+    SPP_HIP(hipMemcpyAsync(taps, temps_host, 4 * off, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_circ_sum1, dim3(1), dim3(MF_THREADS), 0, st, data, n, ng, S);
+    SPP_LAUNCHED("k_circ_sum1");
+    hipLaunchKernelGGL(k_template_conv,
+                       dim3(static_cast<unsigned>(spp::ceil_div(n, CT_B)), static_cast<unsigned>(ntemps)),
+                       dim3(MF_THREADS), 0, st, data, n, ng, S, taps, dtd, convs);
+    SPP_LAUNCHED("k_template_conv");
+    SPP_HIP(hipStreamSynchronize(st));
+    return SPP_OK;
+}

@@ -1,0 +1,128 @@
+"""SIGPROC filterbank reader (mirrors ``sigpyproc.readers.FilReader``)."""
+
+from __future__ import annotations
+
+import logging
+from pathlib import Path
+
+import numpy as np
+
+from sigpyproc3_amd.base import Filterbank
+from sigpyproc3_amd.header import Header
+from sigpyproc3_amd.io.fileio import FileReader, allocate_buffer
+
+logger = logging.getLogger(__name__)
+
+
+def plan_blocks(nsamps: int, gulp: int, skipback: int) -> tuple[int, list[tuple[int, int, int]]]:
+    """The read schedule of ``read_plan`` (readers.py:184-216).
+
+    Returns ``(gulp, [(ii, nsamps_read, skip_samples), ...])``: ``gulp`` is clipped to
+    ``nsamps``, each full read is followed by a seek back of ``skipback`` samples, and a
+    last partial read shorter than ``skipback`` is merged into the previous one.
+    """
+    gulp = min(nsamps, gulp)
+    skipback = abs(skipback)
+    if skipback >= gulp:
+        msg = f"readsamps ({gulp}) must be > skipback ({skipback})"
+        raise ValueError(msg)
+    nreads, lastread = divmod(nsamps, gulp - skipback)
+    if lastread < skipback:
+        nreads -= 1
+        lastread = nsamps - nreads * (gulp - skipback)
+    blocks = [(ii, gulp, -skipback) for ii in range(nreads)]
+    if lastread != 0:
+        blocks.append((nreads, lastread, 0))
+    return gulp, blocks
+
+
+class FilReader(Filterbank):
+    """Read SIGPROC filterbank file(s) (readers.py:33-233)."""
+
+    def __init__(self, filenames, *, check_contiguity: bool = True) -> None:
+        if isinstance(filenames, str | Path):
+            filenames = [filenames]
+        self._filenames = filenames
+        self._header = Header.from_sigproc(filenames, check_contiguity=check_contiguity)
+        self._file = FileReader(self.header.stream_info, mode="r", nbits=self.header.nbits)
+        super().__init__()
+
+    @property
+    def header(self) -> Header:
+        return self._header
+
+    @property
+    def filename(self):
+        return self._filenames[0]
+
+    @property
+    def bitsinfo(self):
+        return self._file.bitsinfo
+
+    @property
+    def chan_stride(self) -> float:
+        """Bytes per channel sample (readers.py:88-91)."""
+        return self.bitsinfo.itemsize / self.bitsinfo.bitfact
+
+    @property
+    def samp_stride(self) -> int:
+        """Bytes per time sample (readers.py:93-96)."""
+        return int(self.header.nchans * self.chan_stride)
+
+    def read_plan(self, gulp: int = 16384, start: int = 0, nsamps: int | None = None,
+                  skipback: int = 0, description: str | None = None, *, quiet: bool = False,
+                  allocator=None):
+        """Yield ``(nsamps_read, ii, data)`` gulps of unpacked samples (readers.py:171-233).
+
+        Same schedule, buffers, allocator hook and errors as the reference; the unpack
+        of 1/2/4-bit data runs on the GPU.
+        """
+        for nsamps_r, ii, raw, unpacked in self._gulps(gulp, start, nsamps, skipback,
+                                                       allocator=allocator, unpack=True):
+            yield nsamps_r, ii, unpacked if unpacked is not None else raw
+
+    def read_plan_packed(self, gulp: int = 16384, start: int = 0, nsamps: int | None = None,
+                         skipback: int = 0, *, allocator=None):
+        """Like ``read_plan`` but yields the packed bytes of each gulp (no unpack)."""
+        for nsamps_r, ii, raw, _ in self._gulps(gulp, start, nsamps, skipback,
+                                                allocator=allocator, unpack=False):
+            yield nsamps_r, ii, raw
+
+    def _gulps(self, gulp, start, nsamps, skipback, *, allocator, unpack: bool):
+        if nsamps is None:
+            nsamps = self.header.nsamples - start
+        gulp, blocks = plan_blocks(nsamps, gulp, skipback)
+        allocator = bytearray if allocator is None else allocator
+        read_buffer = allocate_buffer(allocator, gulp * self.samp_stride)
+        raw_all = np.frombuffer(read_buffer, dtype=np.uint8)
+        do_unpack = unpack and self.bitsinfo.unpack
+        unpack_buffer = None
+        if do_unpack:
+            unpack_buffer = allocate_buffer(allocator, gulp * self.header.nchans)
+            data_all = np.frombuffer(unpack_buffer, dtype=self.bitsinfo.dtype)
+        elif self.bitsinfo.unpack:
+            data_all = None
+        else:
+            data_all = np.frombuffer(read_buffer, dtype=self.bitsinfo.dtype)
+        self._file.seek(start * self.samp_stride)
+        nchans = self.header.nchans
+        for ii, nrows, skip in blocks:
+            block = nrows * nchans
+            logger.debug("read_plan: block %d/%d, %d elements, skipback=%d", ii, len(blocks), block, skip)
+            if do_unpack:
+                nbytes = self._file.creadinto(read_buffer, unpack_buffer)
+            else:
+                nbytes = self._file.creadinto_packed(read_buffer)
+            expected = int(block * self.chan_stride)
+            if nbytes != expected:
+                msg = f"Unexpected number of bytes read from file {nbytes} (actual) != {expected} (expected)"
+                raise ValueError(msg)
+            if skip != 0:
+                self._file.seek(int(skip * nchans * self.chan_stride), whence=1)
+            raw = raw_all[:expected]
+            if do_unpack:
+                yield nrows, ii, raw, data_all[:block]
+            elif data_all is not None:
+                yield nrows, ii, raw, data_all[:block]
+            else:
+                yield nrows, ii, raw, None

@@ -1,0 +1,298 @@
+"""GPU parity: every HIP entry point against the CPU oracle, bit-exact for integer work.
+
+All calls go through the C ABI (``libspp_hip.so``) via the package's drop-in API.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import c_oracle
+from oracle import spp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+UNPACK_KAT = {
+    (1, "big"): [0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0, 1, 0, 1, 1, 1],
+    (1, "little"): [1, 1, 1, 0, 0, 0, 0, 0, 1, 1, 1, 0, 1, 0, 0, 0],
+    (2, "big"): [0, 0, 1, 3, 0, 1, 1, 3],
+    (2, "little"): [3, 1, 0, 0, 3, 1, 1, 0],
+    (4, "big"): [0, 7, 1, 7],
+    (4, "little"): [7, 0, 7, 1],
+}
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+@pytest.mark.parametrize(("nbits", "order"), list(UNPACK_KAT))
+def test_unpack_kat(nbits, order):
+    from sigpyproc3_amd.core import kernels
+
+    expected = np.array(UNPACK_KAT[(nbits, order)], dtype=np.uint8)
+    out = np.empty_like(expected)
+    getattr(kernels, f"unpack{nbits}_8_{order}")(np.array([7, 23], dtype=np.uint8), out)
+    np.testing.assert_array_equal(out, expected)
+
+
+@pytest.mark.parametrize("nbits", [1, 2, 4])
+@pytest.mark.parametrize("order", ["big", "little"])
+@pytest.mark.parametrize(("size", "offset"), [(1 << 20, 0), (1001, 0), (4099, 3), (7, 1)])
+def test_unpack_random_vs_oracle(nbits, order, size, offset, rng):
+    from sigpyproc3_amd.io import bits
+
+    torch = _torch()
+    a = rng.integers(0, 256, size + offset, dtype=np.uint8)
+    src = torch.from_numpy(a).cuda()[offset:]
+    out = torch.empty(size * (8 // nbits) + 1, dtype=torch.uint8, device="cuda")[1:]
+    bits.unpack(src, nbits, out, bitorder=order)
+    np.testing.assert_array_equal(out.cpu().numpy(), O.unpack(a[offset:], nbits, order))
+    host = bits.unpack(a[offset:], nbits, bitorder=order)
+    np.testing.assert_array_equal(host, O.unpack(a[offset:], nbits, order))
+
+
+def test_dedisperse_operator_u8_and_negative_delays(rng):
+    from sigpyproc3_amd.core import kernels
+
+    nsamps, nchans = 6000, 100
+    s = rng.integers(0, 256, nsamps * nchans, dtype=np.uint8)
+    for d in (np.sort(rng.integers(0, 700, nchans))[::-1].astype(np.int32),
+              rng.integers(-300, 300, nchans).astype(np.int32)):
+        a = np.full(nsamps + 50, -1, np.float32)
+        b = a.copy()
+        kernels.dedisperse(s, a, d, nchans, nsamps, 50)
+        c_oracle.dedisperse(s, b, d, nchans, nsamps, 50)
+        np.testing.assert_array_equal(a, b)
+
+
+def test_dedisperse_operator_f32(rng):
+    from sigpyproc3_amd.core import kernels
+
+    nsamps, nchans = 3000, 40
+    s = rng.normal(size=nsamps * nchans).astype(np.float32)
+    d = rng.integers(-100, 100, nchans).astype(np.int32)
+    a = np.zeros(nsamps, np.float32)
+    b = np.zeros(nsamps, np.float32)
+    kernels.dedisperse(s, a, d, nchans, nsamps, 0)
+    c_oracle.dedisperse(s, b, d, nchans, nsamps, 0)
+    # same float64 sum in channel order, rounded once to float32
+    np.testing.assert_array_equal(a, b)
+
+
+def test_notebook_kat_on_gpu(filfiles):
+    # docs/tutorials/quickstart.ipynb:296-417
+    from sigpyproc3_amd import FilReader
+
+    tim = FilReader(filfiles["tutorial"]).dedisperse(30)
+    assert tim.data.dtype == np.float32 and tim.header.nchans == 1 and tim.header.dm == 30
+    assert tim.data.size == 187503
+    assert float(tim.data.sum()) == 19636992.0
+    assert f"{float(tim.data.sum()):.6e}" == "1.963699e+07"
+    assert (tim.data.max(), tim.data.min(), np.median(tim.data)) == (121.0, 88.0, 105.0)
+
+
+@pytest.mark.parametrize("key", ["1bit", "2bit", "4bit", "8bit", "tutorial"])
+@pytest.mark.parametrize("ref", ["fch1", "fmin", "fcenter"])
+def test_filterbank_dedisperse_vs_oracle(filfiles, key, ref):
+    from sigpyproc3_amd import FilReader
+
+    files = filfiles[key] if isinstance(filfiles[key], list) else [filfiles[key]]
+    pieces = [O.read_sigproc(f) for f in files]
+    hdr = pieces[0][0]
+    stream = np.concatenate([O.unpacked_stream(h, r) for h, r in pieces])
+    nsamples = sum(h["nsamples"] for h, _ in pieces)
+    fil = FilReader(filfiles[key])
+    rf = O.ref_frequency(ref, hdr["fch1"], hdr["foff"], hdr["nchans"])
+    for dm in (0, 37.5, 250):
+        d = O.compute_dmdelays(O.chan_freqs(hdr["nchans"], hdr["foff"], hdr["fch1"]), dm, hdr["tsamp"], rf)
+        if 2 * int(np.abs(d).max()) >= nsamples:
+            continue
+        for gulp in (16384, 1000):
+            tim = fil.dedisperse(dm, ref, gulp=gulp)
+            ref_tim = O.filterbank_dedisperse(stream, nsamples, hdr["nchans"], d, gulp=gulp)
+            np.testing.assert_array_equal(tim.data, ref_tim)
+
+
+def test_filterbank_dedisperse_start_nsamps(filfiles):
+    from sigpyproc3_amd import FilReader
+
+    h, raw = O.read_sigproc(filfiles["4bit"])
+    s = O.unpacked_stream(h, raw)
+    fil = FilReader(filfiles["4bit"])
+    d = fil.header.get_dmdelays(60)
+    tim = fil.dedisperse(60, gulp=700, start=100, nsamps=3000)
+    ref = O.filterbank_dedisperse(s, h["nsamples"], h["nchans"], d, gulp=700, start=100, nsamps=3000)
+    np.testing.assert_array_equal(tim.data, ref)
+
+
+def test_filterbank_subband_vs_oracle(filfiles, tmp_path):
+    # tests/test_base.py:163-168 (header) + values against the oracle
+    from sigpyproc3_amd import FilReader
+
+    fil = FilReader(filfiles["4bit"])
+    h, raw = O.read_sigproc(filfiles["4bit"])
+    s = O.unpacked_stream(h, raw)
+    for dm, nsub in ((0, 52), (120, 16), (120, 1)):
+        out = fil.subband(dm=dm, nsub=nsub, outfile_name=str(tmp_path / f"s{dm}_{nsub}.fil"), gulp=900)
+        new = FilReader(out)
+        assert new.header.nchans == nsub and new.header.nbits == 32
+        d = fil.header.get_dmdelays(dm)
+        ref = O.filterbank_subband(s, h["nsamples"], h["nchans"], d, nsub, gulp=900)
+        hdr2, raw2 = O.read_sigproc(out)
+        got = np.frombuffer(raw2.tobytes(), dtype=np.float32)
+        np.testing.assert_array_equal(got, ref)
+        if dm == 0:
+            assert new.header.nsamples == fil.header.nsamples
+
+
+def test_subband_operator_vs_c(rng):
+    from sigpyproc3_amd.core import kernels
+
+    nsamps, nchans, nsub = 5000, 96, 12
+    s = rng.integers(0, 16, nsamps * nchans, dtype=np.uint8)
+    d = rng.integers(-40, 400, nchans).astype(np.int32)
+    md = int(np.abs(d).max())
+    cts = (np.arange(nchans) // (nchans // nsub)).astype(np.int32)
+    a = np.full((nsamps - md) * nsub, 7.0, np.float32)
+    b = np.zeros_like(a)
+    kernels.subband(s, a, d, cts, md, nchans, nsub, nsamps)
+    c_oracle.subband(s, b, d, cts, md, nchans, nsub, nsamps)
+    np.testing.assert_array_equal(a, b)
+
+
+def _sweep_case(rng, nsamps, nchans, nbits, dms, foff, tsamp, fch1=1500.0):
+    from sigpyproc3_amd import _lib
+    from sigpyproc3_amd.sweep import SweepPlan
+
+    order = O.DEFAULT_BITORDER.get(nbits, "big")
+    samples = rng.integers(0, 1 << nbits, nsamps * nchans, dtype=np.uint8)
+    packed = samples if nbits == 8 else O.pack(samples, nbits, order)
+    f = O.chan_freqs(nchans, foff, fch1)
+    table = np.atleast_2d(O.compute_dmdelays(f, dms, tsamp, fch1)).astype(np.int32)
+    plan = SweepPlan(table, nsamps, nbits, 1 if order == "big" else 0)
+    plane = plan.execute(_lib.to_device(packed)).cpu().numpy()
+    ref = c_oracle.sweep(samples, nchans, table, ld=plane.shape[1])
+    for i in range(table.shape[0]):
+        n = int(plan.row_len[i])
+        assert n == nsamps - table[i].max()
+        np.testing.assert_array_equal(plane[i, :n], ref[i, :n], err_msg=f"row {i}")
+        assert not plane[i, n:].any()
+    return plan
+
+
+@pytest.mark.parametrize("nbits", [1, 2, 4, 8])
+@pytest.mark.parametrize("nchans", [64, 336, 1000])
+def test_sweep_vs_oracle(nbits, nchans, rng):
+    _sweep_case(rng, 20000 + 333, nchans, nbits, np.arange(0, 61, 1.5), -300.0 / nchans, 256e-6)
+
+
+def test_sweep_wide_spread_tiles(rng):
+    # DM steps so large that tiles hold fewer than 16 rows; ragged last tile
+    plan = _sweep_case(rng, 30000, 128, 2, np.arange(0, 2000, 90.0), -2.0, 64e-6)
+    assert plan.info.ndm_tiles > 2
+
+
+def test_sweep_wide_accumulators(rng):
+    # 4-bit with > 4368 channels and 8-bit with > 257 channels use 32-bit accumulators
+    _sweep_case(rng, 4096 + 7, 4400, 4, np.array([0.0, 3.0, 7.0]), -0.05, 64e-6)
+    _sweep_case(rng, 5000, 1024, 8, np.arange(0, 20.0), -0.3, 64e-6)
+
+
+def test_sweep_rows_equal_filterbank_dedisperse(filfiles):
+    from sigpyproc3_amd import FilReader
+
+    fil = FilReader(filfiles["tutorial"])
+    dms = np.arange(0, 61, 1.0)
+    res = fil.dedisperse_sweep(dms)
+    for i in (0, 17, 30, 60):
+        ts = res.timeseries(i)
+        ref = fil.dedisperse(dms[i])
+        np.testing.assert_array_equal(ts.data, ref.data)
+        assert ts.header.nsamples == ref.header.nsamples
+
+
+def test_running_normalise_vs_oracle(rng):
+    from sigpyproc3_amd.sweep import running_normalise
+
+    nrows, ld = 5, 30000
+    x = rng.integers(0, 5000, (nrows, ld)).astype(np.float32)
+    row_len = np.array([30000, 29000, 4096, 5000, 12345])
+    for w in (4096, 1001, 64):
+        z = running_normalise(x, row_len, w)
+        for i in range(nrows):
+            n = row_len[i]
+            ref = O.running_normalise(x[i, :n], w)
+            np.testing.assert_allclose(z[i, :n], ref, rtol=2e-7, atol=1e-6)
+
+
+def test_boxcar_mf_vs_oracle(rng):
+    from sigpyproc3_amd.sweep import boxcar_mf
+
+    nrows, ld = 6, 20000
+    z = rng.normal(size=(nrows, ld)).astype(np.float32)
+    z[2, 5000:5016] += 3.0
+    z[4, 100:108] += 4.0
+    row_len = np.array([20000, 19500, 17001, 9000, 2048, 4097])
+    widths = [1, 2, 4, 8, 16, 32, 64, 128, 256]
+    snr, peak, widx = boxcar_mf(z, row_len, widths)
+    for i in range(nrows):
+        n = row_len[i]
+        conv = O.boxcar_conv_closed(z[i, :n], widths).astype(np.float32)
+        it, pk, v = O.mf_argmax(conv)
+        assert (widx[i], peak[i]) == (it, pk)
+        assert abs(snr[i] - v) <= 1e-5 * abs(v) + 1e-5 * np.sqrt(widths[it])
+
+
+@pytest.mark.parametrize("kind", ["boxcar", "gaussian", "lorentzian"])
+def test_convolve_templates_vs_oracle(kind, rng):
+    from sigpyproc3_amd.core import kernels
+    from sigpyproc3_amd.core.filters import MatchedFilter, Template
+
+    z = rng.normal(size=3001).astype(np.float32)
+    if kind == "boxcar":
+        widths = MatchedFilter.get_box_width_spacing(32, 1.5)
+    else:
+        widths = np.geomspace(1, 32, 9)
+    bank = [getattr(Template, f"gen_{kind}")(w) for w in widths]
+    convs = kernels.convolve_templates(z, [t.data for t in bank], [t.ref_bin for t in bank])
+    for k, t in enumerate(bank):
+        ref = O.template_conv_closed(z, t.data, t.ref_bin)
+        np.testing.assert_allclose(convs[k], ref, rtol=1e-5, atol=1e-5)
+        fft = O.convolve_templates_fft(z, [t.data], [t.ref_bin])[0]
+        np.testing.assert_allclose(convs[k], fft, rtol=1e-5, atol=1e-5)
+
+
+def _pulse_data():
+    # tests/test_filters.py:10-25
+    rng = np.random.default_rng(42)
+    x = np.linspace(-10, 10, 1000, dtype=np.float32)
+    return np.exp(-0.5 * (x**2)) + rng.normal(0, 0.1, 1000).astype(np.float32)
+
+
+@pytest.mark.parametrize("temp_kind", ["boxcar", "gaussian", "lorentzian"])
+def test_matched_filter_end_to_end(temp_kind):
+    # tests/test_filters.py:163-168
+    from sigpyproc3_amd.core.filters import MatchedFilter
+
+    mf = MatchedFilter(_pulse_data(), temp_kind=temp_kind)
+    assert mf.snr > 5
+    assert 450 < mf.peak_bin < 550
+    assert mf.convs.shape == (len(mf.temp_widths), 1000)
+    assert mf.best_model.shape == (1000,)
+    start, end = mf.on_pulse
+    assert 0 <= start < end <= 1000
+
+
+def test_matched_filter_detection_and_noise():
+    # tests/test_filters.py:71-93
+    from sigpyproc3_amd.core.filters import MatchedFilter
+
+    mf = MatchedFilter(_pulse_data(), temp_kind="gaussian")
+    assert abs(mf.peak_bin - 500) <= 5 and mf.snr > 10
+    rng = np.random.default_rng(42)
+    noise = rng.normal(loc=5, scale=2, size=1000).astype(np.float32)
+    assert MatchedFilter(noise).snr < 6

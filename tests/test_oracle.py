@@ -1,0 +1,298 @@
+"""Pin the CPU oracle against the reference's known answers and against mathematics.
+
+Each function of ``oracle.spp_oracle`` is checked here against something other than its
+own formula: the reference's golden vectors (tests/test_kernels.py:12-41 unpack KAT, the
+quickstart notebook's dedisperse KAT), library routines (np.unpackbits, scipy's
+pocketfft good size, numpy FFTs), and brute force on small inputs.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.fft
+
+from oracle import c_oracle
+from oracle import spp_oracle as O
+
+# tests/test_kernels.py:12-41 (input [7, 23])
+UNPACK_KAT = {
+    (1, "big"): [0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0, 1, 0, 1, 1, 1],
+    (1, "little"): [1, 1, 1, 0, 0, 0, 0, 0, 1, 1, 1, 0, 1, 0, 0, 0],
+    (2, "big"): [0, 0, 1, 3, 0, 1, 1, 3],
+    (2, "little"): [3, 1, 0, 0, 3, 1, 1, 0],
+    (4, "big"): [0, 7, 1, 7],
+    (4, "little"): [7, 0, 7, 1],
+}
+
+
+@pytest.mark.parametrize(("nbits", "order"), list(UNPACK_KAT))
+def test_unpack_kat(nbits, order):
+    out = O.unpack(np.array([7, 23], dtype=np.uint8), nbits, order)
+    np.testing.assert_array_equal(out, np.array(UNPACK_KAT[(nbits, order)], dtype=np.uint8))
+
+
+@pytest.mark.parametrize("order", ["big", "little"])
+def test_unpack_1bit_is_unpackbits(order, rng):
+    a = rng.integers(0, 256, 4096, dtype=np.uint8)
+    np.testing.assert_array_equal(O.unpack(a, 1, order), np.unpackbits(a, bitorder=order))
+
+
+@pytest.mark.parametrize("nbits", [1, 2, 4])
+@pytest.mark.parametrize("order", ["big", "little"])
+def test_pack_unpack_roundtrip(nbits, order, rng):
+    a = rng.integers(0, 256, 2048, dtype=np.uint8)
+    np.testing.assert_array_equal(O.pack(O.unpack(a, nbits, order), nbits, order), a)
+
+
+def test_c_unpack_matches(rng):
+    a = rng.integers(0, 256, 10000, dtype=np.uint8)
+    for nbits in (1, 2, 4):
+        for order in ("big", "little"):
+            np.testing.assert_array_equal(c_oracle.unpack(a, nbits, order), O.unpack(a, nbits, order))
+
+
+def test_delays_invariants():
+    # tests/test_params.py:9-37: zero delay at the reference channel, int32, shapes
+    freqs = np.arange(1000, dtype=np.float32) * (-1.0) + 1400
+    d = O.compute_dmdelays(freqs, 100, 0.001, float(freqs[0]))
+    assert d.dtype == np.int32 and d.shape == (1000,) and d[0] == 0
+    dv = O.compute_dmdelays(freqs, np.linspace(0, 100, 50), 0.001, float(freqs[0]))
+    assert dv.shape == (50, 1000)
+    np.testing.assert_equal(dv[:, 0], 0)
+    assert (np.diff(dv, axis=1) >= 0).all()  # delays grow towards low frequencies
+
+
+@pytest.mark.parametrize(("dm", "maxdelay"), [(0, 0), (100, 0.8), (5000, 40.4)])
+def test_delays_seconds_vs_reference_test(dm, maxdelay):
+    # tests/test_header.py:35-41 on parkes_4bit (832 ch, fch1 4030, foff -4, tsamp 512 us)
+    f = O.chan_freqs(832, -4.0, 4030.0)
+    d = O.compute_dmdelays(f, dm, 0.000512, 4030.0, in_samples=False)
+    np.testing.assert_allclose(d.max(), maxdelay, atol=0.1)
+
+
+@pytest.mark.parametrize("ref", ["fch1", "fmax", "fmin", "fcenter"])
+def test_delays_within_one_sample_of_float64(ref):
+    f = O.chan_freqs(4096, -0.0732421875, 1550.0)
+    rf = O.ref_frequency(ref, 1550.0, -0.0732421875, 4096)
+    dms = np.arange(0, 2048, 7)
+    d32 = O.compute_dmdelays(f, dms, 64e-6, rf)
+    d64 = O.compute_dmdelays_f64(f, dms, 64e-6, rf)
+    assert np.abs(d32 - d64).max() <= 1
+    assert (d32 == d64).mean() > 0.99
+
+
+def test_notebook_kat(datadir):
+    # docs/tutorials/quickstart.ipynb:296-417: FilReader("tutorial.fil").dedisperse(30)
+    hdr, raw = O.read_sigproc(datadir / "tutorial.fil")
+    s = O.unpacked_stream(hdr, raw)
+    d = O.compute_dmdelays(O.chan_freqs(hdr["nchans"], hdr["foff"], hdr["fch1"]), 30, hdr["tsamp"],
+                           hdr["fch1"])
+    assert int(d.max()) == 17
+    tim = O.filterbank_dedisperse(s, hdr["nsamples"], hdr["nchans"], d)
+    assert tim.size == 187503
+    # the notebook prints the float32 sum to 7 significant digits
+    assert f"{float(tim.sum()):.6e}" == "1.963699e+07"
+    assert float(tim.sum()) == 19636992.0  # exact integer sum (reproduced)
+    assert (tim.max(), tim.min(), np.median(tim)) == (121.0, 88.0, 105.0)
+
+
+def test_read_plan_blocks_tile_outputs():
+    # Outputs of successive gulps tile [0, nsamps - skip) exactly (base.py:413-420)
+    for nsamps, gulp, skip in [(65536, 16384, 7418), (4096, 512, 200), (187520, 16384, 17),
+                               (1000, 1000, 0), (999, 400, 399)]:
+        if skip >= min(nsamps, gulp):
+            with pytest.raises(ValueError):
+                O.read_plan_blocks(nsamps, gulp, skip)
+            continue
+        g = min(nsamps, gulp)
+        covered = []
+        pos = 0
+        for ii, nr, sk in O.read_plan_blocks(nsamps, gulp, skip):
+            assert nr <= g
+            covered.append((ii * (g - skip), ii * (g - skip) + nr - skip))
+            pos += nr + sk
+        assert pos == nsamps
+        if g < 2 * skip:
+            continue  # dedisperse always reads gulp >= 2 * max_delay (base.py:402)
+        flat = np.concatenate([np.arange(a, b) for a, b in covered])
+        np.testing.assert_array_equal(flat, np.arange(nsamps - skip))
+
+
+def test_read_plan_blocks_reference_example():
+    # cfg1: 6 full reads and one of 11740 samples
+    blocks = O.read_plan_blocks(65536, 16384, 7418)
+    assert [b[1] for b in blocks] == [16384] * 6 + [11740]
+
+
+def _small_stream(rng, nsamps, nchans, nbits):
+    return rng.integers(0, 1 << nbits, nsamps * nchans, dtype=np.uint8)
+
+
+def test_dedisperse_gulp_matches_brute_force(rng):
+    nsamps, nchans = 3000, 37
+    s = _small_stream(rng, nsamps, nchans, 8)
+    d = np.sort(rng.integers(0, 200, nchans)).astype(np.int32)[::-1].copy()
+    tim = O.filterbank_dedisperse(s, nsamps, nchans, d, gulp=500)
+    x = s.reshape(nsamps, nchans).astype(np.float64)
+    md = d.max()
+    brute = np.array([sum(x[t + d[c], c] for c in range(nchans)) for t in range(nsamps - md)])
+    np.testing.assert_array_equal(tim, brute.astype(np.float32))
+
+
+def test_dedisperse_negative_delays_wrap(rng):
+    # numba wraps a negative row index by the gulp length (kernels.py:361)
+    nsamps, nchans = 64, 5
+    s = _small_stream(rng, nsamps, nchans, 8)
+    d = np.array([0, -3, -7, 2, -1], dtype=np.int32)
+    out = np.zeros(nsamps, dtype=np.float32)
+    O.dedisperse_kernel(s, out, d, nchans, nsamps, 0)
+    x = s.reshape(nsamps, nchans).astype(np.float64)
+    for t in range(nsamps - 7):
+        assert out[t] == sum(x[(t + d[c]) % nsamps, c] for c in range(nchans))
+
+
+def test_c_dedisperse_matches_numpy(rng):
+    nsamps, nchans = 5000, 64
+    s = _small_stream(rng, nsamps, nchans, 8)
+    d = rng.integers(-50, 300, nchans).astype(np.int32)
+    a = np.zeros(nsamps, np.float32)
+    b = np.zeros(nsamps, np.float32)
+    O.dedisperse_kernel(s, a, d, nchans, nsamps, 0)
+    c_oracle.dedisperse(s, b, d, nchans, nsamps, 0)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_subband_reduces_to_dedisperse(rng):
+    nsamps, nchans = 2000, 32
+    s = _small_stream(rng, nsamps, nchans, 4)
+    d = np.sort(rng.integers(0, 100, nchans)).astype(np.int32)[::-1].copy()
+    sub = O.filterbank_subband(s, nsamps, nchans, d, 1, gulp=700)
+    tim = O.filterbank_dedisperse(s, nsamps, nchans, d, gulp=700)
+    np.testing.assert_array_equal(sub, tim)
+
+
+def test_subband_brute_and_c(rng):
+    nsamps, nchans, nsub = 900, 24, 6
+    s = _small_stream(rng, nsamps, nchans, 8)
+    d = rng.integers(0, 40, nchans).astype(np.int32)
+    cts = (np.arange(nchans) // (nchans // nsub)).astype(np.int32)
+    md = int(d.max())
+    a = np.zeros((nsamps - md) * nsub, np.float32)
+    O.subband_kernel(s, a, d, cts, md, nchans, nsub, nsamps)
+    x = s.reshape(nsamps, nchans).astype(np.float64)
+    brute = np.zeros((nsamps - md, nsub))
+    for c in range(nchans):
+        brute[:, cts[c]] += x[d[c] : d[c] + nsamps - md, c]
+    np.testing.assert_array_equal(a, brute.ravel().astype(np.float32))
+    b = np.zeros_like(a)
+    c_oracle.subband(s, b, d, cts, md, nchans, nsub, nsamps)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_sweep_rows_are_dedisperse(rng):
+    nsamps, nchans = 4000, 48
+    s = _small_stream(rng, nsamps, nchans, 2)
+    f = O.chan_freqs(nchans, -1.0, 1500.0)
+    table = O.compute_dmdelays(f, np.arange(0, 400, 37), 1e-3, 1500.0)
+    rows = O.sweep_plane(s, nchans, table)
+    plane = c_oracle.sweep(s, nchans, table)
+    for i, r in enumerate(rows):
+        np.testing.assert_array_equal(r, O.filterbank_dedisperse(s, nsamps, nchans, table[i]))
+        np.testing.assert_array_equal(plane[i, : r.size], r)
+
+
+def test_good_size_matches_pocketfft():
+    ns = list(range(1, 5000)) + [509454, 1018893, 2**19 - 14834, 1000003]
+    for n in ns:
+        assert O.good_size_real(n) == scipy.fft.next_fast_len(n, real=True)
+
+
+def test_box_widths():
+    np.testing.assert_array_equal(O.box_width_spacing(256, 2.0), [1, 2, 4, 8, 16, 32, 64, 128, 256])
+    w = O.box_width_spacing(16, 1.2)
+    assert w[0] == 1 and (np.diff(w) > 0).all() and w[-1] <= 16
+
+
+@pytest.mark.parametrize("n", [1000, 4093, 30001])
+def test_boxcar_closed_form_matches_fft(n, rng):
+    z = rng.normal(size=n)
+    widths = [1, 2, 4, 8, 16, 256]
+    fft = O.convolve_templates_fft(z, [np.ones(w) for w in widths], [0] * len(widths))
+    closed = O.boxcar_conv_closed(z, widths)
+    np.testing.assert_allclose(closed, fft, atol=1e-9, rtol=0)
+
+
+def test_template_closed_form_matches_fft(rng):
+    n = 2000
+    z = rng.normal(size=n)
+    x = np.arange(-9, 10)
+    h = np.exp(-0.5 * x**2 / 9.0)
+    fft = O.convolve_templates_fft(z, [h], [9])[0]
+    closed = O.template_conv_closed(z, h, 9)
+    np.testing.assert_allclose(closed, fft, atol=1e-9, rtol=0)
+
+
+def test_boxcar_closed_form_brute_force(rng):
+    n = 50
+    z = rng.normal(size=n)
+    ng = O.good_size_real(n)
+    zt = z[np.arange(ng) % n]
+    for w in (1, 3, 7):
+        out = O.boxcar_conv_closed(z, [w])[0]
+        for t in range(n):
+            s = sum(zt[(t + j) % ng] for j in range(w))
+            ref = (s - w * zt.sum() / ng) / np.sqrt(w * (1 - w / ng))
+            assert abs(out[t] - ref) < 1e-12
+
+
+def test_fft_float32_close_to_closed_form(rng):
+    # the reference computes convs with float32 FFTs; the closed form is the exact value
+    n = 509454 // 16
+    z = rng.normal(size=n).astype(np.float32)
+    widths = [1, 16, 256]
+    f32 = O.convolve_templates_fft(z, [np.ones(w, np.float32) for w in widths], [0] * 3, dtype=np.float32)
+    closed = O.boxcar_conv_closed(z, widths)
+    assert np.abs(f32 - closed).max() < 2e-4
+
+
+def test_zscore_reference_definitions(rng):
+    x = rng.normal(5, 2, 1001).astype(np.float32)
+    z, loc, scale = O.estimate_zscore(x, "median", "iqr")
+    q1, q3 = np.percentile(x.astype(np.float64), [25, 75])
+    assert loc[0] == np.median(x)
+    np.testing.assert_allclose(scale[0], (q3 - q1) / 1.3489795003921634)
+    np.testing.assert_allclose(z, (x - loc[0]) / scale[0], rtol=1e-6)
+
+
+def test_symmetric_pad_matches_numpy(rng):
+    x = rng.normal(size=7)
+    for w in (1, 2, 3, 4, 6, 9, 20):
+        p = O.symmetric_pad_window(x, w)
+        L = w // 2
+        n = x.size
+        idx = np.arange(p.size) - L
+        m = np.mod(idx, 2 * n)
+        sym = np.where(m < n, m, 2 * n - 1 - m)
+        np.testing.assert_array_equal(p, x[sym])
+
+
+def test_running_filter_brute_force(rng):
+    x = rng.normal(size=300)
+    for w in (1, 4, 5, 64):
+        got = O.running_filter_mean(x, w)
+        p = O.symmetric_pad_window(x, w)
+        ref = np.array([p[t : t + w].mean() for t in range(x.size)])
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_running_normalise_brute_force(rng):
+    x = rng.integers(0, 1000, 500).astype(np.float32)
+    w = 64
+    z = O.running_normalise(x, w)
+    p = O.symmetric_pad_window(x, w)
+    for t in range(x.size):
+        win = p[t : t + w]
+        sd = win.std()
+        ref = (x[t] - win.mean()) / (sd if sd > 0 else 1.0)
+        assert abs(float(z[t]) - ref) <= 1e-5 * max(1.0, abs(ref))
