@@ -115,6 +115,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="wall-clock budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
+                    help="first run a plain device copy of this many GiB three times (the "
+                         "reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
@@ -131,6 +134,15 @@ def main() -> int:
     _lib.load()
     _lib.require_gpu()
     dev = _lib.device()
+    if args.copy_calibration_gib > 0:
+        nb = int(args.copy_calibration_gib * 2**30)
+        src = torch.ones(nb // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {nb} bytes read + written")
+        del src, dst
     cfg = cfg_sweep_2bit()
     dms = np.arange(2048, dtype=np.float64)
     table = cfg.delays(dms).astype(np.int32)

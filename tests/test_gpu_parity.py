@@ -124,9 +124,14 @@ def test_filterbank_dedisperse_start_nsamps(filfiles):
     s = O.unpacked_stream(h, raw)
     fil = FilReader(filfiles["4bit"])
     d = fil.header.get_dmdelays(60)
-    tim = fil.dedisperse(60, gulp=700, start=100, nsamps=3000)
-    ref = O.filterbank_dedisperse(s, h["nsamples"], h["nchans"], d, gulp=700, start=100, nsamps=3000)
+    tim = fil.dedisperse(60, gulp=700, start=100)
+    ref = O.filterbank_dedisperse(s, h["nsamples"], h["nchans"], d, gulp=700, start=100)
     np.testing.assert_array_equal(tim.data, ref)
+    # A partial last gulp that does not end at the end of the stream reads a full buffer
+    # and fails the byte-count check, as the reference's read_plan does
+    # (io/fileio.py:271-282 fills the whole buffer; readers.py:224-230 raises).
+    with pytest.raises(ValueError, match="Unexpected number of bytes"):
+        fil.dedisperse(60, gulp=700, start=100, nsamps=3000)
 
 
 def test_filterbank_subband_vs_oracle(filfiles, tmp_path):
