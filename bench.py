@@ -115,6 +115,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="wall-clock budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="stage-2 block shape; -1 = autotune (time every shape, keep the fastest)")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="first run a plain device copy of this many GiB three times (the "
                          "reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -153,6 +155,15 @@ def main() -> int:
     torch.cuda.synchronize()
     log(f"plan: {plan.info.ndm_tiles} DM tiles, max spread {plan.info.max_spread}, "
         f"workspace {plan.info.workspace_bytes / 2**30:.1f} GiB")
+    tune_ms = None
+    if args.variant >= 0:
+        plan.variant = args.variant
+    else:
+        from sigpyproc3_amd import tune
+
+        plan.tune(packed, plane, cache=False)
+        tune_ms = tune.last_times()
+        log(f"autotune: variant {plan.variant}; k_sweep ms per variant {tune_ms}")
     for _ in range(args.warmup):
         plan.execute(packed, plane)
     torch.cuda.synchronize()
@@ -223,6 +234,8 @@ def main() -> int:
         "stages_ms": {"k_build_g": round(float(build_ms.mean()), 3), "k_sweep": round(float(sweep_ms.mean()), 3),
                       "k_build_g_gbs": round(stage1_bytes / (build_ms.mean() * 1e-3) / 1e9, 1)},
         "common_length_value": common / sec_per_step,
+        "sweep_variant": plan.variant,
+        "tune_ms": tune_ms,
         "roofline": roofline,
         "cpu_baseline": None,
     }

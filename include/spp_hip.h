@@ -79,7 +79,8 @@ int spp_subband(const void* in, int nbits, int big, int64_t nchans, int64_t nsam
  * No single reference function: it is the stack over DMs of kernels.dedisperse
  * (core/kernels.py:348-362) outputs, which do not depend on the gulp schedule for
  * non-negative delays. Rows have length row_len[i] = nsamps - max_c delays[i][c];
- * plane entries in [row_len[i], ceil(max_len/1024)*1024) are written as 0.
+ * plane entries in [row_len[i], min(ld, ceil(max_len/T)*T)) are written as 0, with the
+ * time tile T = 2048 for 1/2-bit data and 1024 for 4/8-bit data.
  */
 typedef struct spp_sweep_plan spp_sweep_plan;
 
@@ -90,13 +91,18 @@ typedef struct {
     int64_t total_samples;   /* sum of row lengths (DM-trial samples)               */
     int64_t ndm_tiles;       /* DM tiles the planner formed                         */
     int32_t max_spread;      /* largest per-channel delay spread inside a DM tile   */
-    int32_t mode;            /* inner-loop mode chosen for nbits                    */
+    int32_t mode;            /* field bits (4/8/16) + 32 if 32-bit lanes + 64*variant */
 } spp_sweep_info;
 
 /* Plan on the host from a host delay table int32 [ndm][nchans] (all delays >= 0). */
 int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int64_t nchans,
                           int64_t nsamps, int nbits, int big, spp_sweep_plan** plan);
 int spp_sweep_plan_destroy(spp_sweep_plan* plan);
+/* Block shapes of the stage-2 kernel (DM rows per wave x waves x channels per stage):
+ * 0 = 4x4x16, 1 = 2x8x16 (default), 2 = 4x4x8, 3 = 2x8x8. All give identical results;
+ * spp_sweep_plan_set_variant picks one for a plan (the autotuner times them). */
+int spp_sweep_variant_count(void);
+int spp_sweep_plan_set_variant(spp_sweep_plan* plan, int variant);
 int spp_sweep_plan_info(const spp_sweep_plan* plan, spp_sweep_info* info);
 /* Row lengths (int64 [ndm]) into host memory. */
 int spp_sweep_plan_row_lengths(const spp_sweep_plan* plan, int64_t* out_host);

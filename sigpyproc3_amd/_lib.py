@@ -48,6 +48,8 @@ SIGNATURES: dict[str, tuple[object, list]] = {
     "spp_subband": (_int, [_p, _int, _int, _i64, _i64, _p, _p, _i64, _i64, _p, _i64, _p]),
     "spp_sweep_plan_create": (_int, [_p, _i64, _i64, _i64, _int, _int, ctypes.POINTER(_p)]),
     "spp_sweep_plan_destroy": (_int, [_p]),
+    "spp_sweep_variant_count": (_int, []),
+    "spp_sweep_plan_set_variant": (_int, [_p, _int]),
     "spp_sweep_plan_info": (_int, [_p, ctypes.POINTER(SweepInfo)]),
     "spp_sweep_plan_row_lengths": (_int, [_p, _p]),
     "spp_sweep_plan_upload": (_int, [_p, _p, _i64, _p]),

@@ -2,24 +2,29 @@
 // table, over a whole stream resident in HBM.
 //
 // Stage 1 (k_build_g) turns the packed time-major stream [nsamps][nchans] into a
-// channel-major, time-interleaved word array G[c][j] (uint32):
-//     G[c][j] = x_c[j] | x_c[j+S] << 8 | x_c[j+2S] << 16 | x_c[j+3S] << 24,  S = 256,
-// one byte per sample, zero past the end. A delay shift d of channel c moves every
-// byte of a word together, so an output word (4 samples t, t+S, t+2S, t+3S) is the sum
-// over channels of the aligned words G[c][t + d_i(c)].
+// channel-major, time-interleaved word array G[c][j] (uint32) holding IL samples
+// x_c[j + k*S] (k < IL, S = 256) in fields of 32/IL bits, zero past the end:
+//   1- and 2-bit data: IL = 8 nibble fields; 4- and 8-bit data: IL = 4 byte fields.
+// A delay shift d of channel c moves every field of a word together, so an output word
+// (the IL samples t + k*S) is the sum over channels of the aligned words G[c][t + d_i(c)].
 //
 // Stage 2 (k_sweep): a block owns a DM tile (<= 16 consecutive DM rows whose
-// per-channel delay spread is <= SW_SPREAD_MAX, formed by the host planner) and 1024
-// output samples. For each stage of 16 channels it copies G[c][jb .. jb + 256 + spread]
-// into LDS (jb = t0 + min delay of the tile, rounded down to 4 words), then each wave
-// adds, for its 4 DMs, 4 LDS words per lane per channel (lanes read consecutive words:
-// bank-conflict free). Accumulation is SWAR: for nbits <= 4 the four byte lanes of a
-// uint32 accumulate up to floor(255/maxval) channels before they are widened to 16-bit
-// lanes; for 8-bit data the even/odd bytes go straight to 16-bit lanes. Integer sums
-// are exact, so the float32 output is bit-exact to the reference's float64 sums.
+// per-channel delay spread is <= SW_SPREAD_MAX, formed by the host planner) and IL*256
+// output samples. Channels are processed in stages of CC: the stage's windows
+// G[c][jb .. jb + 256 + spread] (jb = t0 + min delay of the tile, rounded down to 4 words)
+// are prefetched into registers while the previous stage is summed, then stored to the
+// other half of a double-buffered LDS area (one barrier per stage). Each wave owns K DM
+// rows; per channel and DM row a lane reads 4 LDS words (lanes read consecutive words:
+// bank-conflict free) and adds them into SWAR accumulators:
+//   nibble fields: 2 channels per v_add3, widened to byte lanes every GN channels
+//                  (nibbles stay <= 15), then to 16-bit lanes before bytes can overflow;
+//   byte fields (4-bit): widened to 16-bit lanes every stage;
+//   8-bit data: even/odd bytes go straight to 16-bit lanes;
+//   32-bit lanes only when nchans * maxval > 65535.
+// Integer sums are exact, so the float32 output is bit-exact to the reference's float64 sums.
 //
-// Work: nchans adds per DM-trial sample. The sweep is bound by LDS reads (1 byte per
-// sample-add) and VALU, not HBM.
+// Work: nchans adds per DM-trial sample. LDS bytes read per sample-add: 4/IL... i.e. 0.5 for
+// nibble fields, 1 for byte fields. The sweep is bound by LDS reads and VALU, not HBM.
 #include "spp_common.hpp"
 
 #include <algorithm>
@@ -28,81 +33,94 @@
 
 namespace {
 
-constexpr int SW_THREADS = 256;
-constexpr int SW_WAVES = 4;
-constexpr int SW_K = 4;                  // DM rows per wave
-constexpr int SW_DT = SW_WAVES * SW_K;   // DM rows per tile
-constexpr int SW_S = 256;                // interleave stride (words per wave row)
-constexpr int SW_T = 4 * SW_S;           // output samples per block
-constexpr int SW_CC = 16;                // channels per LDS stage
-constexpr int SW_LW = 512;               // LDS words per staged channel
+constexpr int SW_S = 256;                // interleave stride = words per wave row
+constexpr int SW_DT = 16;                // DM rows per tile
+constexpr int SW_LW = 496;               // LDS words per staged channel
 constexpr int SW_SPREAD_MAX = SW_LW - SW_S - 4;
-constexpr int G_BUILD_T = 64;            // words per build block (time)
-constexpr int G_BUILD_C = 64;            // channels per build block
-constexpr int G_BUILD_STRIDE = 68;       // bytes per staged row in the build kernel
+constexpr int G_TJ = 32;                 // words per build block (time)
+constexpr int G_STRIDE = 68;             // bytes per staged row in the build kernel
 
-template <int NBITS, bool BIG>
-__device__ __forceinline__ uint32_t packed_sample(const uint8_t* __restrict__ in, int64_t row,
-                                                  int64_t row_bytes, int64_t c) {
-    if constexpr (NBITS == 8) {
-        return in[row * row_bytes + c];
-    } else {
-        constexpr int PER = 8 / NBITS;
-        const uint32_t b = in[row * row_bytes + c / PER];
-        const int k = static_cast<int>(c % PER);
-        const int shift = BIG ? (8 - (k + 1) * NBITS) : k * NBITS;
-        return (b >> shift) & ((1u << NBITS) - 1u);
-    }
-}
+template <int NBITS>
+struct Fields {
+    static constexpr int IL = NBITS <= 2 ? 8 : 4;          // samples per word
+    static constexpr int FB = 32 / IL;                     // bits per field
+    static constexpr uint32_t MAXV = NBITS == 8 ? 255u : ((1u << NBITS) - 1u);
+    static constexpr int CB = 512 / NBITS;                 // build: channels per 64-byte row chunk
+};
 
-// Stage 1. Block (bx, by): words [bx*64, bx*64+64) of channels [by*64, by*64+64).
+// Stage 1. Block (bx, by): words [bx*32, bx*32+32) of channels [by*CB, by*CB+CB).
 template <int NBITS, bool BIG>
 __global__ __launch_bounds__(256) void k_build_g(const uint8_t* __restrict__ in, int64_t nsamps,
                                                  int64_t nchans, uint32_t* __restrict__ G,
                                                  int64_t gpitch) {
-    __shared__ uint8_t smp[4 * G_BUILD_T * G_BUILD_STRIDE];
+    using F = Fields<NBITS>;
+    constexpr int ROWS = F::IL * G_TJ;
+    __shared__ uint8_t raw[ROWS * G_STRIDE];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int64_t j0 = static_cast<int64_t>(blockIdx.x) * G_BUILD_T;
-    const int64_t c0 = static_cast<int64_t>(blockIdx.y) * G_BUILD_C;
+    const int64_t j0 = static_cast<int64_t>(blockIdx.x) * G_TJ;
+    const int64_t c0 = static_cast<int64_t>(blockIdx.y) * F::CB;
     const int64_t row_bytes = nchans * NBITS / 8;
-    const int64_t c = c0 + lane;
-    // Load rows j0 + r*S + i (r < 4, i < 64): lane = channel, coalesced along the row.
-    for (int rr = wave; rr < 4 * G_BUILD_T; rr += 4) {
-        const int r = rr / G_BUILD_T, i = rr % G_BUILD_T;
-        const int64_t row = j0 + static_cast<int64_t>(r) * SW_S + i;
-        uint32_t v = 0;
-        if (row < nsamps && c < nchans) v = packed_sample<NBITS, BIG>(in, row, row_bytes, c);
-        smp[(r * G_BUILD_T + i) * G_BUILD_STRIDE + lane] = static_cast<uint8_t>(v);
+    const int64_t b0 = c0 * NBITS / 8;
+    // Load the 64-byte row chunks of rows j0 + k*S + i (k < IL, i < 32): lane = byte.
+    for (int r = wave; r < ROWS; r += 4) {
+        const int64_t row = j0 + static_cast<int64_t>(r / G_TJ) * SW_S + (r % G_TJ);
+        uint8_t v = 0;
+        if (row < nsamps && b0 + lane < row_bytes) v = in[row * row_bytes + b0 + lane];
+        raw[r * G_STRIDE + lane] = v;
     }
     __syncthreads();
-    // Write G[c0 + k][j0 + lane]: lane = word, coalesced along the channel row.
-    const int64_t j = j0 + lane;
-    if (j >= gpitch) return;
-    for (int k = wave; k < G_BUILD_C; k += 4) {
-        if (c0 + k >= nchans) break;
-        const int i = lane;
-        const uint32_t w = static_cast<uint32_t>(smp[(0 * G_BUILD_T + i) * G_BUILD_STRIDE + k]) |
-                           (static_cast<uint32_t>(smp[(1 * G_BUILD_T + i) * G_BUILD_STRIDE + k]) << 8) |
-                           (static_cast<uint32_t>(smp[(2 * G_BUILD_T + i) * G_BUILD_STRIDE + k]) << 16) |
-                           (static_cast<uint32_t>(smp[(3 * G_BUILD_T + i) * G_BUILD_STRIDE + k]) << 24);
-        G[(c0 + k) * gpitch + j] = w;
+    // Assemble G[c0 + cl][j0 + i]: lanes = 32 consecutive words of 2 channels.
+    const int i = threadIdx.x & (G_TJ - 1);
+    const int64_t j = j0 + i;
+    for (int cl = threadIdx.x / G_TJ; cl < F::CB; cl += 256 / G_TJ) {
+        const int64_t c = c0 + cl;
+        if (c >= nchans) break;
+        const int byte = (cl * NBITS) / 8;
+        int shift = 0;
+        if constexpr (NBITS < 8) {
+            const int k = cl % (8 / NBITS);
+            shift = BIG ? (8 - (k + 1) * NBITS) : k * NBITS;
+        }
+        uint32_t w = 0;
+#pragma unroll
+        for (int k = 0; k < F::IL; ++k) {
+            const uint32_t v = (static_cast<uint32_t>(raw[(k * G_TJ + i) * G_STRIDE + byte]) >> shift) & F::MAXV;
+            w |= v << (F::FB * k);
+        }
+        if (j < gpitch) G[c * gpitch + j] = w;
     }
 }
 
 // Stage 2.
-template <int NBITS, bool USE32>
-__global__ __launch_bounds__(SW_THREADS) void k_sweep(
+template <int NBITS, bool USE32, int K, int WAVES, int CC, int MINW>
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW))) void k_sweep(
     const uint32_t* __restrict__ G, int64_t gpitch, int64_t nchans,
-    const int32_t* __restrict__ table, const int32_t* __restrict__ base,
+    const int32_t* __restrict__ relw, const int32_t* __restrict__ basew,
     const int32_t* __restrict__ tile_dm0, const int32_t* __restrict__ tile_ndm,
     const int32_t* __restrict__ tile_spread, const int32_t* __restrict__ row_len,
     float* __restrict__ plane, int64_t ld, int64_t ntiles_dm, int64_t nblocks) {
-    constexpr uint32_t MAXV = (NBITS == 8) ? 255u : ((1u << NBITS) - 1u);
-    constexpr int F8 = static_cast<int>(255u / MAXV);     // channels per byte-lane flush
-    constexpr int F16 = static_cast<int>(65535u / MAXV);  // channels per 16-bit-lane flush
-    __shared__ uint32_t lds[SW_CC * SW_LW];
-    __shared__ int32_t srel[SW_DT * SW_CC];
+    using F = Fields<NBITS>;
+    constexpr int IL = F::IL;
+    constexpr uint32_t MAXV = F::MAXV;
+    constexpr int THREADS = WAVES * 64;
+    constexpr int D = K * WAVES;
+    constexpr int T = IL * SW_S;
+    constexpr int CPW = CC / WAVES;                 // channels each wave stages per stage
+    constexpr int NLD = (SW_LW + 255) / 256;        // uint4 loads per lane per staged channel
+    static_assert(CC % WAVES == 0, "whole channels per wave");
+    static_assert(D == SW_DT, "the planner forms tiles of SW_DT rows");
+    static_assert(D * CC <= THREADS, "one stage offset per thread");
+    // Channels summed per nibble flush (nibbles <= 15); stages between byte and 16-bit flushes.
+    constexpr int GN = (IL == 8) ? ((15 / MAXV) >= 8 ? 8 : 4) : CC;
+    static_assert(IL != 8 || (CC % GN == 0 && GN * MAXV <= 15), "nibble flush group");
+    constexpr int SB = (NBITS < 8) ? static_cast<int>(255u / (CC * MAXV)) : 1;
+    constexpr int SW16 = static_cast<int>(65535u / (CC * MAXV));
+    static_assert(NBITS == 8 || SB >= 1, "byte lanes must hold one stage");
+    static_assert(SW16 >= 1, "16-bit lanes must hold one stage");
+
+    __shared__ uint32_t lds[2][CC * SW_LW];
+    __shared__ int32_t srel[2][D * CC];
 
     // XCD-aware order: consecutive blocks on one XCD walk the DM tiles of one time tile.
     const int64_t b = blockIdx.x;
@@ -110,7 +128,7 @@ __global__ __launch_bounds__(SW_THREADS) void k_sweep(
     const int64_t logical = (b % 8) * per_xcd + b / 8;
     if (logical >= nblocks) return;
     const int64_t dt = logical % ntiles_dm;
-    const int64_t t0 = (logical / ntiles_dm) * SW_T;
+    const int64_t t0 = (logical / ntiles_dm) * T;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -118,128 +136,227 @@ __global__ __launch_bounds__(SW_THREADS) void k_sweep(
     const int dm0 = tile_dm0[dt];
     const int ndm_t = tile_ndm[dt];
     const int stage_words = SW_S + tile_spread[dt] + 4;
-    const int32_t* __restrict__ tbase = base + dt * nchans;
+    const int32_t* __restrict__ tbase = basew + dt * nchans;
+    const int nst = static_cast<int>((nchans + CC - 1) / CC);
 
-    uint32_t acc8[SW_K][4];
-    uint32_t acc16[SW_K][4][2];
-    uint32_t acc32[USE32 ? SW_K : 1][USE32 ? 16 : 1];
+    // Accumulators (see the header comment). Unused levels are one register.
+    constexpr int NP = (IL == 8) ? 2 : 1;  // field planes after the first widening
+    uint32_t accN[K][4], accN2[K][4];   // nibble fields (IL == 8), two channel groups
+    uint32_t accB4[K][4];                // byte fields of one stage (4-bit data)
+    uint32_t accB[NBITS < 8 ? K : 1][4][NP];
+    uint32_t accW[K][4][NP][2];
+    uint32_t accD[USE32 ? K : 1][USE32 ? 4 * IL : 1];
 #pragma unroll
-    for (int k = 0; k < SW_K; ++k)
+    for (int k = 0; k < K; ++k)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            acc8[k][q] = 0;
-            acc16[k][q][0] = 0;
-            acc16[k][q][1] = 0;
-        }
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                accN[k][q] = 0;
+                accN2[k][q] = 0;
+                accB4[k][q] = 0;
+                if constexpr (NBITS < 8) accB[k][q][p] = 0;
+                accW[k][q][p][0] = 0;
+                accW[k][q][p][1] = 0;
+            }
     if constexpr (USE32) {
 #pragma unroll
-        for (int k = 0; k < SW_K; ++k)
+        for (int k = 0; k < K; ++k)
 #pragma unroll
-            for (int i = 0; i < 16; ++i) acc32[k][i] = 0;
+            for (int i = 0; i < 4 * IL; ++i) accD[k][i] = 0;
     }
 
-    auto flush8 = [&]() {
+    auto flushN = [&]() {
+        if constexpr (IL == 8) {
 #pragma unroll
-        for (int k = 0; k < SW_K; ++k)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                acc16[k][q][0] += acc8[k][q] & 0x00ff00ffu;
-                acc16[k][q][1] += (acc8[k][q] >> 8) & 0x00ff00ffu;
-                acc8[k][q] = 0;
-            }
-    };
-    auto flush16 = [&]() {
-        if constexpr (USE32) {
-#pragma unroll
-            for (int k = 0; k < SW_K; ++k)
+            for (int k = 0; k < K; ++k)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    acc32[k][q * 4 + 0] += acc16[k][q][0] & 0xffffu;
-                    acc32[k][q * 4 + 1] += acc16[k][q][1] & 0xffffu;
-                    acc32[k][q * 4 + 2] += acc16[k][q][0] >> 16;
-                    acc32[k][q * 4 + 3] += acc16[k][q][1] >> 16;
-                    acc16[k][q][0] = 0;
-                    acc16[k][q][1] = 0;
+                    accB[k][q][0] += accN[k][q] & 0x0f0f0f0fu;
+                    accB[k][q][1] += (accN[k][q] >> 4) & 0x0f0f0f0fu;
+                    accN[k][q] = 0;
                 }
         }
     };
-
-    int since8 = 0, since16 = 0;
-    for (int64_t c0 = 0; c0 < nchans; c0 += SW_CC) {
-        const int ncl = static_cast<int>(min<int64_t>(SW_CC, nchans - c0));
+    auto flushB = [&]() {
         if constexpr (NBITS < 8) {
-            if (since8 + ncl > F8) {
-                flush8();
-                since16 += since8;
-                since8 = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) {
+                        accW[k][q][p][0] += accB[k][q][p] & 0x00ff00ffu;
+                        accW[k][q][p][1] += (accB[k][q][p] >> 8) & 0x00ff00ffu;
+                        accB[k][q][p] = 0;
+                    }
+        }
+    };
+    // Sample index (0..IL-1) of 16-bit lane l of accW[.][.][p][h].
+    auto field_of = [](int p, int h, int l) { return (IL == 8) ? (4 * l + 2 * h + p) : (2 * l + h); };
+    auto flushW = [&]() {
+        if constexpr (USE32) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int p = 0; p < NP; ++p)
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            accD[k][q * IL + field_of(p, h, 0)] += accW[k][q][p][h] & 0xffffu;
+                            accD[k][q * IL + field_of(p, h, 1)] += accW[k][q][p][h] >> 16;
+                            accW[k][q][p][h] = 0;
+                        }
+        }
+    };
+
+    // Stage loaders: global -> registers (prefetch) and registers -> LDS. Wave w stages
+    // channels w, w + WAVES, ... of a stage; lane l loads words 4l + 256i of each window.
+    // The tile base offset of a channel (basew) is loaded one stage before it is used, and
+    // the LDS offsets come from relw = d - basew (so t0, a multiple of 4, drops out).
+    uint4 pf[CPW][NLD];
+    int32_t bw_cur[CPW], bw_nxt[CPW];
+    int32_t my_rel = 0;
+    auto load_bw = [&](int st, int32_t* bw) {
+#pragma unroll
+        for (int i = 0; i < CPW; ++i) {
+            const int64_t c = static_cast<int64_t>(st) * CC + wave + WAVES * i;
+            bw[i] = (st < nst && c < nchans) ? tbase[c] : 0;
+        }
+    };
+    auto prefetch = [&](int st) {
+        const int64_t c0 = static_cast<int64_t>(st) * CC;
+#pragma unroll
+        for (int i = 0; i < CPW; ++i) {
+            const int64_t c = c0 + wave + WAVES * i;
+            const bool live = c < nchans;
+            const uint32_t* __restrict__ src = G + c * gpitch + t0 + bw_cur[i];
+            const int64_t jrem = gpitch - (t0 + bw_cur[i]);  // words left in the channel row
+#pragma unroll
+            for (int r = 0; r < NLD; ++r) {
+                const int m = 4 * lane + 256 * r;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (live && m < stage_words && m < jrem) v = *reinterpret_cast<const uint4*>(src + m);
+                pf[i][r] = v;
+            }
+        }
+        if (tid < D * CC) {
+            const int dmi = tid / CC, cl = tid % CC;
+            int32_t rel = cl * SW_LW;  // a channel past the end reads its zero-filled row
+            const int64_t c = c0 + cl;
+            if (c < nchans) {
+                const int row = dm0 + min(dmi, ndm_t - 1);
+                rel += relw[static_cast<int64_t>(row) * nchans + c];
+            }
+            my_rel = rel;
+        }
+    };
+    auto commit = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < CPW; ++i) {
+            const int cl = wave + WAVES * i;
+#pragma unroll
+            for (int r = 0; r < NLD; ++r) {
+                const int m = 4 * lane + 256 * r;
+                if (m < stage_words) *reinterpret_cast<uint4*>(&lds[buf][cl * SW_LW + m]) = pf[i][r];
+            }
+        }
+        if (tid < D * CC) srel[buf][tid] = my_rel;
+    };
+
+    load_bw(0, bw_cur);
+    load_bw(1, bw_nxt);
+    prefetch(0);
+    commit(0);
+    __syncthreads();
+    int nb = 0, nw = 0;
+    for (int st = 0; st < nst; ++st) {
+        const int buf = st & 1;
+        if (st + 1 < nst) {
+#pragma unroll
+            for (int i = 0; i < CPW; ++i) bw_cur[i] = bw_nxt[i];
+            load_bw(st + 2, bw_nxt);
+            prefetch(st + 1);
+        }
+        const uint32_t* __restrict__ L = lds[buf];
+        const int32_t* __restrict__ R = srel[buf];
+        // Sum the GN channels of group g into acc (nibble or byte fields) or accW (8-bit).
+        auto add_group = [&](int g, uint32_t (&acc)[K][4]) {
+#pragma unroll 2
+            for (int cl = g; cl < g + GN; cl += 2) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const uint32_t* p0 = L + R[(wave * K + k) * CC + cl] + lane;
+                    const uint32_t* p1 = L + R[(wave * K + k) * CC + cl + 1] + lane;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const uint32_t w0 = p0[64 * q], w1 = p1[64 * q];
+                        if constexpr (NBITS < 8) {
+                            acc[k][q] += w0 + w1;
+                        } else {
+                            accW[k][q][0][0] += (w0 & 0x00ff00ffu) + (w1 & 0x00ff00ffu);
+                            accW[k][q][0][1] += ((w0 >> 8) & 0x00ff00ffu) + ((w1 >> 8) & 0x00ff00ffu);
+                        }
+                    }
+                }
+            }
+        };
+        if constexpr (IL == 8 && (CC / GN) % 2 == 0) {
+            // Two nibble groups, widened together (one v_add3 per byte plane and word).
+#pragma unroll 1
+            for (int g = 0; g < CC; g += 2 * GN) {
+                add_group(g, accN);
+                add_group(g + GN, accN2);
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        accB[k][q][0] += (accN[k][q] & 0x0f0f0f0fu) + (accN2[k][q] & 0x0f0f0f0fu);
+                        accB[k][q][1] += ((accN[k][q] >> 4) & 0x0f0f0f0fu) + ((accN2[k][q] >> 4) & 0x0f0f0f0fu);
+                        accN[k][q] = 0;
+                        accN2[k][q] = 0;
+                    }
+            }
+        } else if constexpr (IL == 8) {
+#pragma unroll 1
+            for (int g = 0; g < CC; g += GN) {
+                add_group(g, accN);
+                flushN();
+            }
+        } else {
+            add_group(0, accB4);
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if constexpr (NBITS < 8) accB[k][q][0] += accB4[k][q];
+                    accB4[k][q] = 0;
+                }
+        }
+        if constexpr (NBITS < 8) {
+            if (++nb == SB) {
+                flushB();
+                nb = 0;
             }
         }
         if constexpr (USE32) {
-            if (since16 + since8 + ncl > F16) {
-                if constexpr (NBITS < 8) {
-                    flush8();
-                    since16 += since8;
-                    since8 = 0;
-                }
-                flush16();
-                since16 = 0;
+            if (++nw == SW16) {
+                flushB();
+                nb = 0;
+                flushW();
+                nw = 0;
             }
         }
-        __syncthreads();  // the previous stage's LDS reads are done
-        {
-            // Per-(DM, channel) LDS word offsets of this stage.
-            const int dmi = tid / SW_CC, cl = tid % SW_CC;
-            if (cl < ncl) {
-                const int64_t c = c0 + cl;
-                const int64_t jb4 = (t0 + tbase[c]) & ~int64_t(3);
-                const int row = dm0 + min(dmi, ndm_t - 1);
-                srel[tid] = static_cast<int32_t>(t0 - jb4) + table[static_cast<int64_t>(row) * nchans + c] +
-                            cl * SW_LW;
-            }
-            // Stage G[c][jb4 .. jb4 + stage_words) of this stage's channels.
-            for (int s = wave; s < ncl; s += SW_WAVES) {
-                const int64_t c = c0 + s;
-                const int64_t jb4 = (t0 + tbase[c]) & ~int64_t(3);
-                const uint32_t* __restrict__ src = G + c * gpitch + jb4;
-                for (int m = lane * 4; m < stage_words; m += 256) {
-                    uint4 v = make_uint4(0u, 0u, 0u, 0u);
-                    if (jb4 + m < gpitch) v = *reinterpret_cast<const uint4*>(src + m);
-                    *reinterpret_cast<uint4*>(&lds[s * SW_LW + m]) = v;
-                }
-            }
-        }
+        if (st + 1 < nst) commit(buf ^ 1);
         __syncthreads();
-        for (int cl = 0; cl < ncl; ++cl) {
-#pragma unroll
-            for (int k = 0; k < SW_K; ++k) {
-                const int rel = srel[(wave * SW_K + k) * SW_CC + cl];
-                const uint32_t* p = lds + rel + lane;
-                const uint32_t w0 = p[0], w1 = p[64], w2 = p[128], w3 = p[192];
-                if constexpr (NBITS < 8) {
-                    acc8[k][0] += w0;
-                    acc8[k][1] += w1;
-                    acc8[k][2] += w2;
-                    acc8[k][3] += w3;
-                } else {
-                    acc16[k][0][0] += w0 & 0x00ff00ffu;
-                    acc16[k][0][1] += (w0 >> 8) & 0x00ff00ffu;
-                    acc16[k][1][0] += w1 & 0x00ff00ffu;
-                    acc16[k][1][1] += (w1 >> 8) & 0x00ff00ffu;
-                    acc16[k][2][0] += w2 & 0x00ff00ffu;
-                    acc16[k][2][1] += (w2 >> 8) & 0x00ff00ffu;
-                    acc16[k][3][0] += w3 & 0x00ff00ffu;
-                    acc16[k][3][1] += (w3 >> 8) & 0x00ff00ffu;
-                }
-            }
-        }
-        if constexpr (NBITS < 8) since8 += ncl; else since16 += ncl;
     }
-    if constexpr (NBITS < 8) flush8();
+    flushB();
 
-    // Store: sample (k, q, r) is t0 + lane + 64q + 256r of DM row dm0 + wave*SW_K + k.
+    // Store: field f of word (lane + 64q) is sample t0 + lane + 64q + f*S of DM row dm0 + wave*K + k.
 #pragma unroll
-    for (int k = 0; k < SW_K; ++k) {
-        const int idx = wave * SW_K + k;
+    for (int k = 0; k < K; ++k) {
+        const int idx = wave * K + k;
         if (idx >= ndm_t) break;
         const int64_t row = dm0 + idx;
         const int64_t len = row_len[row];
@@ -247,13 +364,17 @@ __global__ __launch_bounds__(SW_THREADS) void k_sweep(
 #pragma unroll
         for (int q = 0; q < 4; ++q)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t t = t0 + lane + 64 * q + SW_S * r;
-                const uint32_t h = acc16[k][q][r & 1];
-                uint32_t v = (r < 2) ? (h & 0xffffu) : (h >> 16);
-                if constexpr (USE32) v += acc32[k][q * 4 + r];
-                if (t < ld) orow[t] = (t < len) ? static_cast<float>(v) : 0.f;
-            }
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int l = 0; l < 2; ++l) {
+                        const int f = field_of(p, h, l);
+                        const int64_t t = t0 + lane + 64 * q + static_cast<int64_t>(SW_S) * f;
+                        uint32_t v = (l == 0) ? (accW[k][q][p][h] & 0xffffu) : (accW[k][q][p][h] >> 16);
+                        if constexpr (USE32) v += accD[k][q * IL + f];
+                        if (t < ld) orow[t] = (t < len) ? static_cast<float>(v) : 0.f;
+                    }
     }
 }
 
@@ -266,13 +387,15 @@ struct spp_sweep_plan {
     int nbits = 8, big = 0;
     std::vector<int32_t> table;  // [ndm][nchans]
     std::vector<int32_t> tile_dm0, tile_ndm, tile_spread;
-    std::vector<int32_t> base;   // [ntiles][nchans]
+    std::vector<int32_t> base;   // [ntiles][nchans] = min delay of the tile, rounded down to 4
     std::vector<int32_t> row_len;
     int64_t max_len = 0, min_len = 0, total = 0;
     int32_t max_spread = 0;
     bool use32 = false;
+    int variant = 0;  // k_sweep block shape (launch_sweep)
     int64_t gpitch = 0;
-    int64_t off_table = 0, off_base = 0, off_dm0 = 0, off_ndm = 0, off_spread = 0, off_rowlen = 0,
+    std::vector<int32_t> relw;   // [ndm][nchans] = delay - (tile base & ~3): LDS word offset
+    int64_t off_relw = 0, off_base = 0, off_dm0 = 0, off_ndm = 0, off_spread = 0, off_rowlen = 0,
             off_g = 0, ws_bytes = 0;
     const void* uploaded_to = nullptr;
 };
@@ -327,6 +450,7 @@ extern "C" int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int
         p->total += len;
     }
     // Greedy DM tiles: consecutive rows while the per-channel delay spread fits LDS.
+    p->relw.resize(static_cast<size_t>(ndm * nchans));
     std::vector<int32_t> lo(nchans), hi(nchans);
     for (int64_t i = 0; i < ndm;) {
         const int32_t* r0 = &p->table[i * nchans];
@@ -350,15 +474,19 @@ extern "C" int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int
         p->tile_dm0.push_back(static_cast<int32_t>(i));
         p->tile_ndm.push_back(static_cast<int32_t>(j - i));
         p->tile_spread.push_back(spread);
-        p->base.insert(p->base.end(), lo.begin(), lo.end());
+        for (int64_t r = i; r < j; ++r)
+            for (int64_t c = 0; c < nchans; ++c)
+                p->relw[r * nchans + c] = p->table[r * nchans + c] - (lo[c] & ~int32_t(3));
+        for (int64_t c = 0; c < nchans; ++c) p->base.push_back(lo[c] & ~int32_t(3));
         p->max_spread = std::max(p->max_spread, spread);
         i = j;
     }
     p->use32 = static_cast<int64_t>(nchans) * maxv > 65535;
+    p->variant = 1;  // 8 waves x 2 DM rows, 16 channels per stage (tunable per plan)
     p->gpitch = ((nsamps + 63) / 64) * 64;
     const int64_t ntiles = static_cast<int64_t>(p->tile_dm0.size());
     int64_t off = 0;
-    p->off_table = off; off = align256(off + 4 * ndm * nchans);
+    p->off_relw = off; off = align256(off + 4 * ndm * nchans);
     p->off_base = off; off = align256(off + 4 * ntiles * nchans);
     p->off_dm0 = off; off = align256(off + 4 * ntiles);
     p->off_ndm = off; off = align256(off + 4 * ntiles);
@@ -367,6 +495,17 @@ extern "C" int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int
     p->off_g = off; off = align256(off + 4 * nchans * p->gpitch);
     p->ws_bytes = off;
     *plan = p;
+    return SPP_OK;
+}
+
+extern "C" int spp_sweep_variant_count(void) { return 4; }
+
+extern "C" int spp_sweep_plan_set_variant(spp_sweep_plan* p, int variant) {
+    spp::clear_error();
+    SPP_REQUIRE(p != nullptr, "null plan");
+    SPP_REQUIRE(variant >= 0 && variant < spp_sweep_variant_count(), "variant %d outside [0, %d)", variant,
+                spp_sweep_variant_count());
+    p->variant = variant;
     return SPP_OK;
 }
 
@@ -384,7 +523,7 @@ extern "C" int spp_sweep_plan_info(const spp_sweep_plan* p, spp_sweep_info* info
     info->total_samples = p->total;
     info->ndm_tiles = static_cast<int64_t>(p->tile_dm0.size());
     info->max_spread = p->max_spread;
-    info->mode = (p->nbits == 8 ? 16 : 8) + (p->use32 ? 32 : 0);
+    info->mode = (p->nbits <= 2 ? 4 : (p->nbits == 4 ? 8 : 16)) + (p->use32 ? 32 : 0) + 64 * p->variant;
     return SPP_OK;
 }
 
@@ -404,7 +543,7 @@ extern "C" int spp_sweep_plan_upload(spp_sweep_plan* p, void* workspace, int64_t
     hipStream_t st = spp::as_stream(stream);
     char* ws = static_cast<char*>(workspace);
     const int64_t ntiles = static_cast<int64_t>(p->tile_dm0.size());
-    SPP_HIP(hipMemcpyAsync(ws + p->off_table, p->table.data(), 4 * p->ndm * p->nchans,
+    SPP_HIP(hipMemcpyAsync(ws + p->off_relw, p->relw.data(), 4 * p->ndm * p->nchans,
                            hipMemcpyHostToDevice, st));
     SPP_HIP(hipMemcpyAsync(ws + p->off_base, p->base.data(), 4 * ntiles * p->nchans,
                            hipMemcpyHostToDevice, st));
@@ -422,24 +561,24 @@ namespace {
 
 template <int NBITS, bool BIG>
 int launch_build(const uint8_t* packed, const spp_sweep_plan* p, uint32_t* G, hipStream_t st) {
-    dim3 grid(static_cast<unsigned>(spp::ceil_div(p->gpitch, G_BUILD_T)),
-              static_cast<unsigned>(spp::ceil_div(p->nchans, G_BUILD_C)));
+    dim3 grid(static_cast<unsigned>(spp::ceil_div(p->gpitch, G_TJ)),
+              static_cast<unsigned>(spp::ceil_div(p->nchans, Fields<NBITS>::CB)));
     hipLaunchKernelGGL((k_build_g<NBITS, BIG>), grid, dim3(256), 0, st, packed, p->nsamps, p->nchans, G,
                        p->gpitch);
     SPP_LAUNCHED("k_build_g");
     return SPP_OK;
 }
 
-template <int NBITS, bool USE32>
-int launch_sweep(const spp_sweep_plan* p, char* ws, float* plane, int64_t ld, hipStream_t st) {
+template <int NBITS, bool USE32, int K, int WAVES, int CC, int MINW>
+int launch_sweep_v(const spp_sweep_plan* p, char* ws, float* plane, int64_t ld, hipStream_t st) {
     const int64_t ntiles_dm = static_cast<int64_t>(p->tile_dm0.size());
-    const int64_t ntiles_t = spp::ceil_div(p->max_len, SW_T);
+    const int64_t ntiles_t = spp::ceil_div(p->max_len, int64_t(Fields<NBITS>::IL) * SW_S);
     const int64_t nblocks = ntiles_dm * ntiles_t;
     const int64_t grid = spp::ceil_div(nblocks, 8) * 8;
     SPP_REQUIRE(grid <= 0x7fffffffLL, "sweep grid too large");
-    hipLaunchKernelGGL((k_sweep<NBITS, USE32>), dim3(static_cast<unsigned>(grid)), dim3(SW_THREADS), 0,
-                       st, reinterpret_cast<const uint32_t*>(ws + p->off_g), p->gpitch, p->nchans,
-                       reinterpret_cast<const int32_t*>(ws + p->off_table),
+    hipLaunchKernelGGL((k_sweep<NBITS, USE32, K, WAVES, CC, MINW>), dim3(static_cast<unsigned>(grid)),
+                       dim3(WAVES * 64), 0, st, reinterpret_cast<const uint32_t*>(ws + p->off_g),
+                       p->gpitch, p->nchans, reinterpret_cast<const int32_t*>(ws + p->off_relw),
                        reinterpret_cast<const int32_t*>(ws + p->off_base),
                        reinterpret_cast<const int32_t*>(ws + p->off_dm0),
                        reinterpret_cast<const int32_t*>(ws + p->off_ndm),
@@ -448,6 +587,17 @@ int launch_sweep(const spp_sweep_plan* p, char* ws, float* plane, int64_t ld, hi
                        nblocks);
     SPP_LAUNCHED("k_sweep");
     return SPP_OK;
+}
+
+// Block shapes: (DM rows per wave, waves, channels per stage); all tile SW_DT rows.
+template <int NBITS, bool USE32>
+int launch_sweep(const spp_sweep_plan* p, char* ws, float* plane, int64_t ld, hipStream_t st) {
+    switch (p->variant) {
+        case 1: return launch_sweep_v<NBITS, USE32, 2, 8, 16, 4>(p, ws, plane, ld, st);
+        case 2: return launch_sweep_v<NBITS, USE32, 4, 4, 8, 3>(p, ws, plane, ld, st);
+        case 3: return launch_sweep_v<NBITS, USE32, 2, 8, 8, 4>(p, ws, plane, ld, st);
+        default: return launch_sweep_v<NBITS, USE32, 4, 4, 16, 2>(p, ws, plane, ld, st);
+    }
 }
 
 }  // namespace

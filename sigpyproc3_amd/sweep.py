@@ -49,6 +49,32 @@ class SweepPlan:
         return int(self.info.max_len)
 
     @property
+    def variant(self) -> int:
+        """Block shape of the stage-2 kernel (see ``spp_sweep_plan_set_variant``)."""
+        return int(self.info.mode) // 64
+
+    @variant.setter
+    def variant(self, v: int) -> None:
+        lib = _lib.load()
+        _lib.check(lib.spp_sweep_plan_set_variant(self._handle, int(v)))
+        _lib.check(lib.spp_sweep_plan_info(self._handle, ctypes.byref(self.info)))
+
+    @staticmethod
+    def variant_count() -> int:
+        return int(_lib.load().spp_sweep_variant_count())
+
+    def tune(self, packed, plane=None, repeats: int = 2, cache: bool = True, stream=None) -> int:
+        """Time every stage-2 block shape on this plan and keep the fastest.
+
+        The choice is cached in a JSON file keyed by (device, nbits, nchans, ndm, nsamps)
+        (``SPP_TUNE_CACHE``, default ``~/.cache/sigpyproc3_amd/sweep_tune.json``); a cached
+        key is applied without timing. Returns the chosen variant.
+        """
+        from sigpyproc3_amd import tune
+
+        return tune.tune_plan(self, packed, plane=plane, repeats=repeats, cache=cache, stream=stream)
+
+    @property
     def total_samples(self) -> int:
         """DM-trial samples one execution produces (sum of row lengths)."""
         return int(self.info.total_samples)

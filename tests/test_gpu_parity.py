@@ -169,7 +169,7 @@ def test_subband_operator_vs_c(rng):
     np.testing.assert_array_equal(a, b)
 
 
-def _sweep_case(rng, nsamps, nchans, nbits, dms, foff, tsamp, fch1=1500.0):
+def _sweep_case(rng, nsamps, nchans, nbits, dms, foff, tsamp, fch1=1500.0, variants=(None,)):
     from sigpyproc3_amd import _lib
     from sigpyproc3_amd.sweep import SweepPlan
 
@@ -179,20 +179,25 @@ def _sweep_case(rng, nsamps, nchans, nbits, dms, foff, tsamp, fch1=1500.0):
     f = O.chan_freqs(nchans, foff, fch1)
     table = np.atleast_2d(O.compute_dmdelays(f, dms, tsamp, fch1)).astype(np.int32)
     plan = SweepPlan(table, nsamps, nbits, 1 if order == "big" else 0)
-    plane = plan.execute(_lib.to_device(packed)).cpu().numpy()
-    ref = c_oracle.sweep(samples, nchans, table, ld=plane.shape[1])
-    for i in range(table.shape[0]):
-        n = int(plan.row_len[i])
-        assert n == nsamps - table[i].max()
-        np.testing.assert_array_equal(plane[i, :n], ref[i, :n], err_msg=f"row {i}")
-        assert not plane[i, n:].any()
+    dev = _lib.to_device(packed)
+    ref = c_oracle.sweep(samples, nchans, table, ld=plan.max_len)
+    for v in variants:
+        if v is not None:
+            plan.variant = v
+        plane = plan.execute(dev).cpu().numpy()
+        for i in range(table.shape[0]):
+            n = int(plan.row_len[i])
+            assert n == nsamps - table[i].max()
+            np.testing.assert_array_equal(plane[i, :n], ref[i, :n], err_msg=f"row {i} variant {v}")
+            assert not plane[i, n:].any()
     return plan
 
 
 @pytest.mark.parametrize("nbits", [1, 2, 4, 8])
 @pytest.mark.parametrize("nchans", [64, 336, 1000])
 def test_sweep_vs_oracle(nbits, nchans, rng):
-    _sweep_case(rng, 20000 + 333, nchans, nbits, np.arange(0, 61, 1.5), -300.0 / nchans, 256e-6)
+    _sweep_case(rng, 20000 + 333, nchans, nbits, np.arange(0, 61, 1.5), -300.0 / nchans, 256e-6,
+                variants=(0, 1, 2, 3))
 
 
 def test_sweep_wide_spread_tiles(rng):
@@ -203,8 +208,11 @@ def test_sweep_wide_spread_tiles(rng):
 
 def test_sweep_wide_accumulators(rng):
     # 4-bit with > 4368 channels and 8-bit with > 257 channels use 32-bit accumulators
-    _sweep_case(rng, 4096 + 7, 4400, 4, np.array([0.0, 3.0, 7.0]), -0.05, 64e-6)
-    _sweep_case(rng, 5000, 1024, 8, np.arange(0, 20.0), -0.3, 64e-6)
+    _sweep_case(rng, 4096 + 7, 4400, 4, np.array([0.0, 3.0, 7.0]), -0.05, 64e-6, variants=(0, 1, 2, 3))
+    _sweep_case(rng, 5000, 1024, 8, np.arange(0, 20.0), -0.3, 64e-6, variants=(0, 1, 2, 3))
+    # 2-bit above 21845 channels and 1-bit above 65535 channels overflow 16-bit lanes
+    _sweep_case(rng, 3000, 22000, 2, np.array([0.0, 1.0, 2.0]), -0.01, 64e-6, variants=(0, 1, 2, 3))
+    _sweep_case(rng, 2100, 65600, 1, np.array([0.0, 1.0]), -0.004, 64e-6, variants=(1, 3))
 
 
 def test_sweep_rows_equal_filterbank_dedisperse(filfiles):
