@@ -107,6 +107,75 @@ def cpu_baseline(packed_dev, cfg, table, row_len, plane, budget_s: float):
     }
 
 
+def bench_serve(args) -> int:
+    """16 streams (8-bit, 4096 ch), Poisson gulp arrivals at 0.5/0.9/1.2 x measured capacity,
+    gulps of U[2^14, 2^18] new samples overlapped by each stream's largest delay, DM lists of
+    U[512, 4096] trials up to DM U[500, 3000]. Latency = arrival -> sweep complete."""
+    import torch
+
+    from sigpyproc3_amd import _lib, serve
+    from sigpyproc3_amd.params import compute_dmdelays
+
+    _lib.load()
+    _lib.require_gpu()
+    nstreams, lo, hi = 16, 1 << 14, 1 << 18
+    nchans, fch1, foff, tsamp = 4096, 1550.0, -0.0732421875, 64e-6
+    freqs = np.arange(nchans, dtype=np.float32) * np.float32(foff) + np.float32(fch1)
+    rng = np.random.default_rng(5)
+    states, dm_lists = [], []
+    t_plan = time.perf_counter()
+    for _s in range(nstreams):
+        ndm = int(rng.integers(512, 4097))
+        dms = np.linspace(0.0, float(rng.uniform(500.0, 3000.0)), ndm)
+        table = np.atleast_2d(compute_dmdelays(freqs, dms, tsamp, fch1)).astype(np.int32)
+        states.append(serve.StreamState(table, 8, 0, hi))
+        dm_lists.append(ndm)
+    t_plan = time.perf_counter() - t_plan
+    source = serve.make_source(hi, nchans)
+    log(f"serve: {nstreams} streams, DM lists {dm_lists}, overlaps {[s.overlap for s in states]}, "
+        f"plans {t_plan:.1f} s, device memory {torch.cuda.memory_allocated() / 2**30:.1f} GiB")
+    for st in states:                       # first gulp of every stream (no overlap) + warmup
+        st.submit(source, hi)
+        st.submit(source, lo)
+    torch.cuda.synchronize()
+
+    def samples_of(sched):
+        return sum(len(states[g.stream].table) * g.nsamps for g in sched)
+
+    burst = serve.poisson_schedule(nstreams, args.serve_gulps, 0.0, lo, hi, seed=11)
+    wall = serve.run_arrivals(states, source, burst)
+    capacity = len(burst) / wall
+    cap_samples = samples_of(burst) / wall
+    log(f"serve: capacity {capacity:.1f} gulps/s ({cap_samples:.3e} DM-trial samples/s) over {len(burst)} gulps")
+    runs = {}
+    for frac in (0.5, 0.9, 1.2):
+        sched = serve.poisson_schedule(nstreams, args.serve_gulps, frac * capacity, lo, hi, seed=int(frac * 100))
+        wall = serve.run_arrivals(states, source, sched)
+        summ = serve.latency_summary(sched)
+        summ["offered_gulps_per_s"] = frac * capacity
+        summ["dm_trial_samples_per_s"] = samples_of(sched) / wall
+        runs[f"{frac}x"] = summ
+        log(f"serve {frac}x: p50 {summ['p50_ms']:.1f} ms, p99 {summ['p99_ms']:.1f} ms")
+    result = {
+        "metric": "p50/p99 gulp latency at 0.5/0.9/1.2 x capacity (16-stream sweep service)",
+        "value": runs["0.9x"]["dm_trial_samples_per_s"],
+        "unit": UNIT,
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "16 streams x 8-bit x 4096 ch (1550 MHz, foff -0.0732, 64 us), gulps "
+                               "U[2^14, 2^18] + overlap, DM lists U[512, 4096] to DM U[500, 3000]",
+                   "dm_lists": dm_lists, "overlaps": [s.overlap for s in states]},
+        "capacity_gulps_per_s": capacity,
+        "capacity_dm_trial_samples_per_s": cap_samples,
+        "latency": runs,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,11 +189,18 @@ def main() -> int:
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="first run a plain device copy of this many GiB three times (the "
                          "reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--workload", choices=("sweep", "serve"), default="sweep",
+                    help="sweep: the 2-bit 2048-DM sweep (the headline line); serve: the "
+                         "16-stream gulp service with p50/p99 latency")
+    ap.add_argument("--serve-gulps", type=int, default=160, help="gulps per serve run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
               "(the sweep shards by DM list across GPUs; see DESIGN.md)", file=sys.stderr)
         return 1
+
+    if args.workload == "serve":
+        return bench_serve(args)
 
     import torch
 

@@ -120,6 +120,12 @@ int spp_sweep_build(const spp_sweep_plan* plan, const uint8_t* packed, void* wor
                     int64_t workspace_bytes, void* stream);
 int spp_sweep_run(const spp_sweep_plan* plan, float* plane, int64_t ld, void* workspace,
                   int64_t workspace_bytes, void* stream);
+/* spp_sweep_execute on the first nsamps samples only (max delay of the table < nsamps <= the
+ * plan's nsamps): rows are nsamps - max_c delays[i][c] long. One plan then serves gulps of any
+ * length up to its own (a stream processed in overlapping gulps of varying size). */
+int spp_sweep_execute_n(const spp_sweep_plan* plan, const uint8_t* packed, int64_t nsamps,
+                        float* plane, int64_t ld, void* workspace, int64_t workspace_bytes,
+                        void* stream);
 
 /*
  * Running normalisation of DM series (no reference counterpart: the running mean/std

@@ -48,11 +48,12 @@ struct Fields {
     static constexpr int CB = 512 / NBITS;                 // build: channels per 64-byte row chunk
 };
 
-// Stage 1. Block (bx, by): words [bx*32, bx*32+32) of channels [by*CB, by*CB+CB).
+// Stage 1. Block (bx, by): words [bx*32, bx*32+32) of channels [by*CB, by*CB+CB); only the
+// first glen words of each channel row are written (the sweep reads no further).
 template <int NBITS, bool BIG>
 __global__ __launch_bounds__(256) void k_build_g(const uint8_t* __restrict__ in, int64_t nsamps,
                                                  int64_t nchans, uint32_t* __restrict__ G,
-                                                 int64_t gpitch) {
+                                                 int64_t gpitch, int64_t glen) {
     using F = Fields<NBITS>;
     constexpr int ROWS = F::IL * G_TJ;
     __shared__ uint8_t raw[ROWS * G_STRIDE];
@@ -88,39 +89,64 @@ __global__ __launch_bounds__(256) void k_build_g(const uint8_t* __restrict__ in,
             const uint32_t v = (static_cast<uint32_t>(raw[(k * G_TJ + i) * G_STRIDE + byte]) >> shift) & F::MAXV;
             w |= v << (F::FB * k);
         }
-        if (j < gpitch) G[c * gpitch + j] = w;
+        if (j < glen) G[c * gpitch + j] = w;
     }
 }
 
-// Stage 2.
+// Stage 2. Per stage of CC channels: nibble data (IL == 8) is summed in NG groups of GN
+// channels (GN * maxval <= 15 keeps nibbles exact); each pair of groups is widened to byte
+// lanes with one v_add3 per byte plane. Byte data (4-bit) is summed over the stage in byte
+// lanes, then widened; 8-bit data goes straight to 16-bit lanes.
+template <int NBITS, int CC>
+struct StageGroups {
+    static constexpr int IL = Fields<NBITS>::IL;
+    static constexpr uint32_t MAXV = Fields<NBITS>::MAXV;
+    static constexpr int pick() {
+        if (IL != 8) return 1;
+        for (int ng = 2; ng <= CC; ng += 2)
+            if (CC % ng == 0 && static_cast<uint32_t>(CC / ng) * MAXV <= 15u) return ng;
+        return -1;
+    }
+    static constexpr int NG = pick();
+    static constexpr int GN = NG > 0 ? CC / NG : 1;
+};
+
+// Word 3 of a raw buffer resource on gfx9-family GPUs: 32-bit data format, no swizzle.
+constexpr int kBufferFlags = 0x00020000;
+
+// Bytes 1 and 3 of x into the low bytes of its two 16-bit lanes.
+__device__ __forceinline__ uint32_t odd_bytes(uint32_t x) { return __builtin_amdgcn_perm(x, x, 0x0c030c01u); }
+
 template <int NBITS, bool USE32, int K, int WAVES, int CC, int MINW>
 __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW))) void k_sweep(
-    const uint32_t* __restrict__ G, int64_t gpitch, int64_t nchans,
+    const uint32_t* __restrict__ G, int64_t gpitch, int64_t glen, int64_t nchans,
     const int32_t* __restrict__ relw, const int32_t* __restrict__ basew,
     const int32_t* __restrict__ tile_dm0, const int32_t* __restrict__ tile_ndm,
     const int32_t* __restrict__ tile_spread, const int32_t* __restrict__ row_len,
-    float* __restrict__ plane, int64_t ld, int64_t ntiles_dm, int64_t nblocks) {
+    float* __restrict__ plane, int64_t ld, int64_t shrink, int64_t ntiles_dm, int64_t nblocks) {
     using F = Fields<NBITS>;
+    using SG = StageGroups<NBITS, CC>;
     constexpr int IL = F::IL;
     constexpr uint32_t MAXV = F::MAXV;
     constexpr int THREADS = WAVES * 64;
     constexpr int D = K * WAVES;
     constexpr int T = IL * SW_S;
-    constexpr int CPW = CC / WAVES;                 // channels each wave stages per stage
-    constexpr int NLD = (SW_LW + 255) / 256;        // uint4 loads per lane per staged channel
-    static_assert(CC % WAVES == 0, "whole channels per wave");
+    constexpr int CPW = (CC + WAVES - 1) / WAVES;      // channels each wave stages per stage
+    constexpr int NLD = (SW_LW + 255) / 256;            // uint4 loads per lane per staged channel
+    constexpr int BUFW = CC * SW_LW;                    // words per LDS buffer
+    static_assert(SW_LW % 4 == 0, "whole uint4 chunks");
     static_assert(D == SW_DT, "the planner forms tiles of SW_DT rows");
-    static_assert(D * CC <= THREADS, "one stage offset per thread");
-    // Channels summed per nibble flush (nibbles <= 15); stages between byte and 16-bit flushes.
-    constexpr int GN = (IL == 8) ? ((15 / MAXV) >= 8 ? 8 : 4) : CC;
-    static_assert(IL != 8 || (CC % GN == 0 && GN * MAXV <= 15), "nibble flush group");
-    constexpr int SB = (NBITS < 8) ? static_cast<int>(255u / (CC * MAXV)) : 1;
+    constexpr int NRL = (D * CC + THREADS - 1) / THREADS;  // stage offsets per thread
+    static_assert(IL != 8 || (SG::NG > 0 && SG::NG % 2 == 0), "nibble groups come in pairs");
+    static_assert(NBITS != 4 || CC * 15 <= 255, "4-bit stage sums must fit byte lanes");
+    constexpr int NG = SG::NG, GN = SG::GN;
+    // Stages between byte -> 16-bit widenings (nibble data), and between 16 -> 32-bit ones.
+    constexpr int SB = (IL == 8) ? static_cast<int>(255u / (CC * MAXV)) : 1;
     constexpr int SW16 = static_cast<int>(65535u / (CC * MAXV));
-    static_assert(NBITS == 8 || SB >= 1, "byte lanes must hold one stage");
-    static_assert(SW16 >= 1, "16-bit lanes must hold one stage");
+    static_assert(SB >= 1 && SW16 >= 1, "lanes must hold one stage");
 
-    __shared__ uint32_t lds[2][CC * SW_LW];
-    __shared__ int32_t srel[2][D * CC];
+    __shared__ uint32_t lds[2 * BUFW];
+    __shared__ int32_t srel[2][D * CC];   // byte offset into lds of (DM row, stage channel)
 
     // XCD-aware order: consecutive blocks on one XCD walk the DM tiles of one time tile.
     const int64_t b = blockIdx.x;
@@ -139,11 +165,8 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
     const int32_t* __restrict__ tbase = basew + dt * nchans;
     const int nst = static_cast<int>((nchans + CC - 1) / CC);
 
-    // Accumulators (see the header comment). Unused levels are one register.
-    constexpr int NP = (IL == 8) ? 2 : 1;  // field planes after the first widening
-    uint32_t accN[K][4], accN2[K][4];   // nibble fields (IL == 8), two channel groups
-    uint32_t accB4[K][4];                // byte fields of one stage (4-bit data)
-    uint32_t accB[NBITS < 8 ? K : 1][4][NP];
+    constexpr int NP = (IL == 8) ? 2 : 1;  // byte planes of nibble data
+    uint32_t accB[IL == 8 ? K : 1][4][NP];
     uint32_t accW[K][4][NP][2];
     uint32_t accD[USE32 ? K : 1][USE32 ? 4 * IL : 1];
 #pragma unroll
@@ -152,10 +175,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
         for (int q = 0; q < 4; ++q)
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
-                accN[k][q] = 0;
-                accN2[k][q] = 0;
-                accB4[k][q] = 0;
-                if constexpr (NBITS < 8) accB[k][q][p] = 0;
+                if constexpr (IL == 8) accB[k][q][p] = 0;
                 accW[k][q][p][0] = 0;
                 accW[k][q][p][1] = 0;
             }
@@ -166,20 +186,8 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
             for (int i = 0; i < 4 * IL; ++i) accD[k][i] = 0;
     }
 
-    auto flushN = [&]() {
-        if constexpr (IL == 8) {
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    accB[k][q][0] += accN[k][q] & 0x0f0f0f0fu;
-                    accB[k][q][1] += (accN[k][q] >> 4) & 0x0f0f0f0fu;
-                    accN[k][q] = 0;
-                }
-        }
-    };
     auto flushB = [&]() {
-        if constexpr (NBITS < 8) {
+        if constexpr (IL == 8) {
 #pragma unroll
             for (int k = 0; k < K; ++k)
 #pragma unroll
@@ -187,7 +195,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
 #pragma unroll
                     for (int p = 0; p < NP; ++p) {
                         accW[k][q][p][0] += accB[k][q][p] & 0x00ff00ffu;
-                        accW[k][q][p][1] += (accB[k][q][p] >> 8) & 0x00ff00ffu;
+                        accW[k][q][p][1] += odd_bytes(accB[k][q][p]);
                         accB[k][q][p] = 0;
                     }
         }
@@ -211,59 +219,76 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
         }
     };
 
-    // Stage loaders: global -> registers (prefetch) and registers -> LDS. Wave w stages
-    // channels w, w + WAVES, ... of a stage; lane l loads words 4l + 256i of each window.
-    // The tile base offset of a channel (basew) is loaded one stage before it is used, and
-    // the LDS offsets come from relw = d - basew (so t0, a multiple of 4, drops out).
+    // Staging: wave w copies channels w, w + WAVES, ... of a stage (channel indices, bounds
+    // and source rows are wave-uniform); lane l loads words 4l + 256i of each window
+    // G[c][t0 + base_c .. + stage_words). Tile bases (basew) are loaded one stage before they
+    // are used; LDS offsets come from relw = d - base (t0, a multiple of 4, drops out).
     uint4 pf[CPW][NLD];
     int32_t bw_cur[CPW], bw_nxt[CPW];
-    int32_t my_rel = 0;
+    int32_t my_rel[NRL];
     auto load_bw = [&](int st, int32_t* bw) {
 #pragma unroll
         for (int i = 0; i < CPW; ++i) {
-            const int64_t c = static_cast<int64_t>(st) * CC + wave + WAVES * i;
-            bw[i] = (st < nst && c < nchans) ? tbase[c] : 0;
+            const int cl = wave + WAVES * i;
+            const int64_t c = static_cast<int64_t>(st) * CC + cl;
+            bw[i] = (st < nst && cl < CC && c < nchans) ? tbase[c] : 0;
         }
     };
     auto prefetch = [&](int st) {
         const int64_t c0 = static_cast<int64_t>(st) * CC;
 #pragma unroll
         for (int i = 0; i < CPW; ++i) {
-            const int64_t c = c0 + wave + WAVES * i;
+            const int cl = wave + WAVES * i;
+            if (cl >= CC) break;
+            const int64_t c = c0 + cl;
             const bool live = c < nchans;
-            const uint32_t* __restrict__ src = G + c * gpitch + t0 + bw_cur[i];
-            const int64_t jrem = gpitch - (t0 + bw_cur[i]);  // words left in the channel row
+            // Row c of G as a buffer of its glen valid words: reads past them return 0 (a
+            // channel past the end has an empty buffer), so no bounds test is needed here.
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<uint32_t*>(G) + (live ? c : 0) * gpitch, 0,
+                live ? static_cast<int>(glen * 4) : 0, kBufferFlags);
+            const uint32_t off = static_cast<uint32_t>(t0 + bw_cur[i]) * 4u + 16u * lane;
 #pragma unroll
             for (int r = 0; r < NLD; ++r) {
-                const int m = 4 * lane + 256 * r;
-                uint4 v = make_uint4(0u, 0u, 0u, 0u);
-                if (live && m < stage_words && m < jrem) v = *reinterpret_cast<const uint4*>(src + m);
-                pf[i][r] = v;
+                if (4 * lane + 256 * r < stage_words) {
+                    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 1024u * r, 0, 0);
+                    pf[i][r] = make_uint4(v[0], v[1], v[2], v[3]);
+                }
             }
         }
-        if (tid < D * CC) {
-            const int dmi = tid / CC, cl = tid % CC;
-            int32_t rel = cl * SW_LW;  // a channel past the end reads its zero-filled row
+#pragma unroll
+        for (int u = 0; u < NRL; ++u) {
+            const int e = tid + THREADS * u;
+            const int dmi = e / CC, cl = e % CC;
+            int32_t rel = cl * SW_LW;  // a channel past the end reads its zero-filled window
             const int64_t c = c0 + cl;
-            if (c < nchans) {
+            if (e < D * CC && c < nchans) {
                 const int row = dm0 + min(dmi, ndm_t - 1);
                 rel += relw[static_cast<int64_t>(row) * nchans + c];
             }
-            my_rel = rel;
+            my_rel[u] = rel;
         }
     };
     auto commit = [&](int buf) {
 #pragma unroll
         for (int i = 0; i < CPW; ++i) {
             const int cl = wave + WAVES * i;
+            if (cl >= CC) break;
 #pragma unroll
             for (int r = 0; r < NLD; ++r) {
                 const int m = 4 * lane + 256 * r;
-                if (m < stage_words) *reinterpret_cast<uint4*>(&lds[buf][cl * SW_LW + m]) = pf[i][r];
+                if (m < stage_words) *reinterpret_cast<uint4*>(&lds[buf * BUFW + cl * SW_LW + m]) = pf[i][r];
             }
         }
-        if (tid < D * CC) srel[buf][tid] = my_rel;
+#pragma unroll
+        for (int u = 0; u < NRL; ++u) {
+            const int e = tid + THREADS * u;
+            if (e < D * CC) srel[buf][e] = 4 * (buf * BUFW + my_rel[u]);
+        }
     };
+
+    const char* __restrict__ Lb = reinterpret_cast<const char*>(lds);
+    const int lane4 = 4 * lane;
 
     load_bw(0, bw_cur);
     load_bw(1, bw_nxt);
@@ -279,62 +304,67 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
             load_bw(st + 2, bw_nxt);
             prefetch(st + 1);
         }
-        const uint32_t* __restrict__ L = lds[buf];
         const int32_t* __restrict__ R = srel[buf];
-        // Sum the GN channels of group g into acc (nibble or byte fields) or accW (8-bit).
-        auto add_group = [&](int g, uint32_t (&acc)[K][4]) {
-#pragma unroll 2
-            for (int cl = g; cl < g + GN; cl += 2) {
+        // The 4 words (lane + 64q) of DM row k, stage channel cl.
+        auto ld4 = [&](int k, int cl, uint32_t (&w)[4]) {
+            const char* p = Lb + R[(wave * K + k) * CC + cl] + lane4;
 #pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const uint32_t* p0 = L + R[(wave * K + k) * CC + cl] + lane;
-                    const uint32_t* p1 = L + R[(wave * K + k) * CC + cl + 1] + lane;
+            for (int q = 0; q < 4; ++q) w[q] = *reinterpret_cast<const uint32_t*>(p + 256 * q);
+        };
+        // Sum of channels [c0, c0 + n) of DM row k, per word.
+        auto group_sum = [&](int k, int c0, int n, uint32_t (&s)[4]) {
+            ld4(k, c0, s);
+#pragma unroll 4
+            for (int j = 1; j < n; ++j) {
+                uint32_t w[4];
+                ld4(k, c0 + j, w);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const uint32_t w0 = p0[64 * q], w1 = p1[64 * q];
-                        if constexpr (NBITS < 8) {
-                            acc[k][q] += w0 + w1;
-                        } else {
-                            accW[k][q][0][0] += (w0 & 0x00ff00ffu) + (w1 & 0x00ff00ffu);
-                            accW[k][q][0][1] += ((w0 >> 8) & 0x00ff00ffu) + ((w1 >> 8) & 0x00ff00ffu);
-                        }
-                    }
-                }
+                for (int q = 0; q < 4; ++q) s[q] += w[q];
             }
         };
-        if constexpr (IL == 8 && (CC / GN) % 2 == 0) {
-            // Two nibble groups, widened together (one v_add3 per byte plane and word).
+        if constexpr (IL == 8) {
 #pragma unroll 1
-            for (int g = 0; g < CC; g += 2 * GN) {
-                add_group(g, accN);
-                add_group(g + GN, accN2);
+            for (int g = 0; g < NG; g += 2)
 #pragma unroll
-                for (int k = 0; k < K; ++k)
+                for (int k = 0; k < K; ++k) {
+                    uint32_t a[4], bb[4];
+                    group_sum(k, g * GN, GN, a);
+                    group_sum(k, (g + 1) * GN, GN, bb);
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        accB[k][q][0] += (accN[k][q] & 0x0f0f0f0fu) + (accN2[k][q] & 0x0f0f0f0fu);
-                        accB[k][q][1] += ((accN[k][q] >> 4) & 0x0f0f0f0fu) + ((accN2[k][q] >> 4) & 0x0f0f0f0fu);
-                        accN[k][q] = 0;
-                        accN2[k][q] = 0;
+                        accB[k][q][0] += (a[q] & 0x0f0f0f0fu) + (bb[q] & 0x0f0f0f0fu);
+                        accB[k][q][1] += ((a[q] >> 4) & 0x0f0f0f0fu) + ((bb[q] >> 4) & 0x0f0f0f0fu);
                     }
-            }
-        } else if constexpr (IL == 8) {
-#pragma unroll 1
-            for (int g = 0; g < CC; g += GN) {
-                add_group(g, accN);
-                flushN();
-            }
-        } else {
-            add_group(0, accB4);
+                    // Keep the scheduler from hoisting every row's LDS reads (register spills).
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+        } else if constexpr (NBITS == 4) {
 #pragma unroll
-            for (int k = 0; k < K; ++k)
+            for (int k = 0; k < K; ++k) {
+                uint32_t a[4];
+                group_sum(k, 0, CC, a);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    if constexpr (NBITS < 8) accB[k][q][0] += accB4[k][q];
-                    accB4[k][q] = 0;
+                    accW[k][q][0][0] += a[q] & 0x00ff00ffu;
+                    accW[k][q][0][1] += odd_bytes(a[q]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll 2
+                for (int cl = 0; cl < CC; cl += 2) {
+                    uint32_t w0[4], w1[4];
+                    ld4(k, cl, w0);
+                    ld4(k, cl + 1, w1);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        accW[k][q][0][0] += (w0[q] & 0x00ff00ffu) + (w1[q] & 0x00ff00ffu);
+                        accW[k][q][0][1] += odd_bytes(w0[q]) + odd_bytes(w1[q]);
+                    }
                 }
         }
-        if constexpr (NBITS < 8) {
+        if constexpr (IL == 8) {
             if (++nb == SB) {
                 flushB();
                 nb = 0;
@@ -359,7 +389,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
         const int idx = wave * K + k;
         if (idx >= ndm_t) break;
         const int64_t row = dm0 + idx;
-        const int64_t len = row_len[row];
+        const int64_t len = row_len[row] - shrink;
         float* __restrict__ orow = plane + row * ld;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -410,7 +440,7 @@ extern "C" int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int
                 "sweep needs nbits in {1, 2, 4, 8}, got %d", nbits);
     SPP_REQUIRE(ndm > 0 && nchans > 0 && nsamps > 0, "ndm, nchans, nsamps must be > 0");
     SPP_REQUIRE((nchans * nbits) % 8 == 0, "nchans * nbits must be a multiple of 8");
-    SPP_REQUIRE(nsamps < (int64_t(1) << 31), "nsamps must be < 2^31");
+    SPP_REQUIRE(nsamps < (int64_t(1) << 30), "nsamps must be < 2^30");
     SPP_REQUIRE(ndm < (int64_t(1) << 31), "ndm must be < 2^31");
     const uint32_t maxv = (nbits == 8) ? 255u : ((1u << nbits) - 1u);
     SPP_REQUIRE(static_cast<double>(nchans) * maxv < 16777216.0,
@@ -559,44 +589,68 @@ extern "C" int spp_sweep_plan_upload(spp_sweep_plan* p, void* workspace, int64_t
 
 namespace {
 
+// One execution of a plan on a stream of n <= plan nsamps samples: every row is shorter by
+// shrink = nsamps - n, and G holds glen valid words per channel.
+struct Extent {
+    int64_t n, glen, shrink, max_len;
+};
+
+Extent extent_of(const spp_sweep_plan* p, int64_t n) {
+    const int64_t glen = std::min(p->gpitch, ((n + 63) / 64) * 64);
+    return Extent{n, glen, p->nsamps - n, p->max_len - (p->nsamps - n)};
+}
+
 template <int NBITS, bool BIG>
-int launch_build(const uint8_t* packed, const spp_sweep_plan* p, uint32_t* G, hipStream_t st) {
-    dim3 grid(static_cast<unsigned>(spp::ceil_div(p->gpitch, G_TJ)),
+int launch_build(const uint8_t* packed, const spp_sweep_plan* p, const Extent& e, uint32_t* G,
+                 hipStream_t st) {
+    dim3 grid(static_cast<unsigned>(spp::ceil_div(e.glen, G_TJ)),
               static_cast<unsigned>(spp::ceil_div(p->nchans, Fields<NBITS>::CB)));
-    hipLaunchKernelGGL((k_build_g<NBITS, BIG>), grid, dim3(256), 0, st, packed, p->nsamps, p->nchans, G,
-                       p->gpitch);
+    hipLaunchKernelGGL((k_build_g<NBITS, BIG>), grid, dim3(256), 0, st, packed, e.n, p->nchans, G,
+                       p->gpitch, e.glen);
     SPP_LAUNCHED("k_build_g");
     return SPP_OK;
 }
 
 template <int NBITS, bool USE32, int K, int WAVES, int CC, int MINW>
-int launch_sweep_v(const spp_sweep_plan* p, char* ws, float* plane, int64_t ld, hipStream_t st) {
+int launch_sweep_v(const spp_sweep_plan* p, const Extent& e, char* ws, float* plane, int64_t ld,
+                   hipStream_t st) {
     const int64_t ntiles_dm = static_cast<int64_t>(p->tile_dm0.size());
-    const int64_t ntiles_t = spp::ceil_div(p->max_len, int64_t(Fields<NBITS>::IL) * SW_S);
+    const int64_t ntiles_t = spp::ceil_div(e.max_len, int64_t(Fields<NBITS>::IL) * SW_S);
     const int64_t nblocks = ntiles_dm * ntiles_t;
     const int64_t grid = spp::ceil_div(nblocks, 8) * 8;
     SPP_REQUIRE(grid <= 0x7fffffffLL, "sweep grid too large");
     hipLaunchKernelGGL((k_sweep<NBITS, USE32, K, WAVES, CC, MINW>), dim3(static_cast<unsigned>(grid)),
                        dim3(WAVES * 64), 0, st, reinterpret_cast<const uint32_t*>(ws + p->off_g),
-                       p->gpitch, p->nchans, reinterpret_cast<const int32_t*>(ws + p->off_relw),
+                       p->gpitch, e.glen, p->nchans, reinterpret_cast<const int32_t*>(ws + p->off_relw),
                        reinterpret_cast<const int32_t*>(ws + p->off_base),
                        reinterpret_cast<const int32_t*>(ws + p->off_dm0),
                        reinterpret_cast<const int32_t*>(ws + p->off_ndm),
                        reinterpret_cast<const int32_t*>(ws + p->off_spread),
-                       reinterpret_cast<const int32_t*>(ws + p->off_rowlen), plane, ld, ntiles_dm,
-                       nblocks);
+                       reinterpret_cast<const int32_t*>(ws + p->off_rowlen), plane, ld, e.shrink,
+                       ntiles_dm, nblocks);
     SPP_LAUNCHED("k_sweep");
     return SPP_OK;
 }
 
 // Block shapes: (DM rows per wave, waves, channels per stage); all tile SW_DT rows.
+// Nibble data (1/2-bit): stages of 10 or 20 channels give 5-channel nibble groups for 2-bit.
 template <int NBITS, bool USE32>
-int launch_sweep(const spp_sweep_plan* p, char* ws, float* plane, int64_t ld, hipStream_t st) {
-    switch (p->variant) {
-        case 1: return launch_sweep_v<NBITS, USE32, 2, 8, 16, 4>(p, ws, plane, ld, st);
-        case 2: return launch_sweep_v<NBITS, USE32, 4, 4, 8, 3>(p, ws, plane, ld, st);
-        case 3: return launch_sweep_v<NBITS, USE32, 2, 8, 8, 4>(p, ws, plane, ld, st);
-        default: return launch_sweep_v<NBITS, USE32, 4, 4, 16, 2>(p, ws, plane, ld, st);
+int launch_sweep(const spp_sweep_plan* p, const Extent& e, char* ws, float* plane, int64_t ld,
+                 hipStream_t st) {
+    if constexpr (NBITS <= 2) {
+        switch (p->variant) {
+            case 1: return launch_sweep_v<NBITS, USE32, 2, 8, 20, 4>(p, e, ws, plane, ld, st);
+            case 2: return launch_sweep_v<NBITS, USE32, 2, 8, 16, 4>(p, e, ws, plane, ld, st);
+            case 3: return launch_sweep_v<NBITS, USE32, 4, 4, 20, 2>(p, e, ws, plane, ld, st);
+            default: return launch_sweep_v<NBITS, USE32, 2, 8, 10, 4>(p, e, ws, plane, ld, st);
+        }
+    } else {
+        switch (p->variant) {
+            case 1: return launch_sweep_v<NBITS, USE32, 2, 8, 16, 4>(p, e, ws, plane, ld, st);
+            case 2: return launch_sweep_v<NBITS, USE32, 4, 4, 8, 3>(p, e, ws, plane, ld, st);
+            case 3: return launch_sweep_v<NBITS, USE32, 2, 8, 8, 4>(p, e, ws, plane, ld, st);
+            default: return launch_sweep_v<NBITS, USE32, 4, 4, 16, 2>(p, e, ws, plane, ld, st);
+        }
     }
 }
 
@@ -611,6 +665,34 @@ int check_exec(const spp_sweep_plan* p, const void* workspace, int64_t workspace
     return SPP_OK;
 }
 
+int build_n(const spp_sweep_plan* p, const uint8_t* packed, const Extent& e, void* workspace,
+            hipStream_t st) {
+    uint32_t* G = reinterpret_cast<uint32_t*>(static_cast<char*>(workspace) + p->off_g);
+    switch (p->nbits * 2 + p->big) {
+        case 2: return launch_build<1, false>(packed, p, e, G, st);
+        case 3: return launch_build<1, true>(packed, p, e, G, st);
+        case 4: return launch_build<2, false>(packed, p, e, G, st);
+        case 5: return launch_build<2, true>(packed, p, e, G, st);
+        case 8: return launch_build<4, false>(packed, p, e, G, st);
+        case 9: return launch_build<4, true>(packed, p, e, G, st);
+        default: return launch_build<8, false>(packed, p, e, G, st);
+    }
+}
+
+int run_n(const spp_sweep_plan* p, const Extent& e, float* plane, int64_t ld, void* workspace,
+          hipStream_t st) {
+    SPP_REQUIRE(plane != nullptr, "null plane");
+    SPP_REQUIRE(ld >= e.max_len, "ld (%lld) < longest row (%lld)", static_cast<long long>(ld),
+                static_cast<long long>(e.max_len));
+    char* ws = static_cast<char*>(workspace);
+    switch (p->nbits) {
+        case 1: return p->use32 ? launch_sweep<1, true>(p, e, ws, plane, ld, st) : launch_sweep<1, false>(p, e, ws, plane, ld, st);
+        case 2: return p->use32 ? launch_sweep<2, true>(p, e, ws, plane, ld, st) : launch_sweep<2, false>(p, e, ws, plane, ld, st);
+        case 4: return p->use32 ? launch_sweep<4, true>(p, e, ws, plane, ld, st) : launch_sweep<4, false>(p, e, ws, plane, ld, st);
+        default: return p->use32 ? launch_sweep<8, true>(p, e, ws, plane, ld, st) : launch_sweep<8, false>(p, e, ws, plane, ld, st);
+    }
+}
+
 }  // namespace
 
 extern "C" int spp_sweep_build(const spp_sweep_plan* p, const uint8_t* packed, void* workspace,
@@ -619,17 +701,7 @@ extern "C" int spp_sweep_build(const spp_sweep_plan* p, const uint8_t* packed, v
     int rc = check_exec(p, workspace, workspace_bytes);
     if (rc) return rc;
     SPP_REQUIRE(packed != nullptr, "null packed stream");
-    hipStream_t st = spp::as_stream(stream);
-    uint32_t* G = reinterpret_cast<uint32_t*>(static_cast<char*>(workspace) + p->off_g);
-    switch (p->nbits * 2 + p->big) {
-        case 2: return launch_build<1, false>(packed, p, G, st);
-        case 3: return launch_build<1, true>(packed, p, G, st);
-        case 4: return launch_build<2, false>(packed, p, G, st);
-        case 5: return launch_build<2, true>(packed, p, G, st);
-        case 8: return launch_build<4, false>(packed, p, G, st);
-        case 9: return launch_build<4, true>(packed, p, G, st);
-        default: return launch_build<8, false>(packed, p, G, st);
-    }
+    return build_n(p, packed, extent_of(p, p->nsamps), workspace, spp::as_stream(stream));
 }
 
 extern "C" int spp_sweep_run(const spp_sweep_plan* p, float* plane, int64_t ld, void* workspace,
@@ -637,17 +709,7 @@ extern "C" int spp_sweep_run(const spp_sweep_plan* p, float* plane, int64_t ld, 
     spp::clear_error();
     int rc = check_exec(p, workspace, workspace_bytes);
     if (rc) return rc;
-    SPP_REQUIRE(plane != nullptr, "null plane");
-    SPP_REQUIRE(ld >= p->max_len, "ld (%lld) < longest row (%lld)", static_cast<long long>(ld),
-                static_cast<long long>(p->max_len));
-    hipStream_t st = spp::as_stream(stream);
-    char* ws = static_cast<char*>(workspace);
-    switch (p->nbits) {
-        case 1: return p->use32 ? launch_sweep<1, true>(p, ws, plane, ld, st) : launch_sweep<1, false>(p, ws, plane, ld, st);
-        case 2: return p->use32 ? launch_sweep<2, true>(p, ws, plane, ld, st) : launch_sweep<2, false>(p, ws, plane, ld, st);
-        case 4: return p->use32 ? launch_sweep<4, true>(p, ws, plane, ld, st) : launch_sweep<4, false>(p, ws, plane, ld, st);
-        default: return p->use32 ? launch_sweep<8, true>(p, ws, plane, ld, st) : launch_sweep<8, false>(p, ws, plane, ld, st);
-    }
+    return run_n(p, extent_of(p, p->nsamps), plane, ld, workspace, spp::as_stream(stream));
 }
 
 extern "C" int spp_sweep_execute(const spp_sweep_plan* p, const uint8_t* packed, float* plane,
@@ -656,4 +718,23 @@ extern "C" int spp_sweep_execute(const spp_sweep_plan* p, const uint8_t* packed,
     int rc = spp_sweep_build(p, packed, workspace, workspace_bytes, stream);
     if (rc) return rc;
     return spp_sweep_run(p, plane, ld, workspace, workspace_bytes, stream);
+}
+
+extern "C" int spp_sweep_execute_n(const spp_sweep_plan* p, const uint8_t* packed, int64_t nsamps,
+                                   float* plane, int64_t ld, void* workspace,
+                                   int64_t workspace_bytes, void* stream) {
+    spp::clear_error();
+    int rc = check_exec(p, workspace, workspace_bytes);
+    if (rc) return rc;
+    SPP_REQUIRE(packed != nullptr, "null packed stream");
+    SPP_REQUIRE(nsamps <= p->nsamps, "nsamps %lld exceeds the plan's %lld", static_cast<long long>(nsamps),
+                static_cast<long long>(p->nsamps));
+    SPP_REQUIRE(nsamps > p->nsamps - p->min_len,
+                "nsamps %lld must exceed the largest delay of the table (%lld)",
+                static_cast<long long>(nsamps), static_cast<long long>(p->nsamps - p->min_len));
+    const Extent e = extent_of(p, nsamps);
+    hipStream_t st = spp::as_stream(stream);
+    rc = build_n(p, packed, e, workspace, st);
+    if (rc) return rc;
+    return run_n(p, e, plane, ld, workspace, st);
 }

@@ -112,6 +112,27 @@ class SweepPlan:
                   _lib.ptr(self.workspace), int(self.info.workspace_bytes), _lib.stream_handle(stream))
         return plane
 
+    def row_len_for(self, nsamps: int) -> np.ndarray:
+        """Row lengths of an execution on the first ``nsamps`` samples."""
+        return self.row_len - (self.nsamps - int(nsamps))
+
+    def execute_n(self, packed, nsamps: int, plane, ld: int | None = None, stream=None):
+        """``execute`` on the first ``nsamps`` samples of ``packed`` (max delay < nsamps <=
+        the plan's nsamps): rows are ``row_len_for(nsamps)`` long. ``plane`` is a contiguous
+        float32 tensor of at least ``ndm * ld`` elements, ``ld`` >= the longest row."""
+        nsamps = int(nsamps)
+        need = nsamps * self.nchans * self.nbits // 8
+        if int(packed.numel()) < need:
+            msg = f"packed stream holds {int(packed.numel())} bytes < {need}"
+            raise ValueError(msg)
+        ld = self.max_len - (self.nsamps - nsamps) if ld is None else int(ld)
+        if int(plane.numel()) < self.ndm * ld or not plane.is_contiguous():
+            msg = f"plane must be a contiguous float32 tensor of >= {self.ndm} x {ld} elements"
+            raise ValueError(msg)
+        _lib.call("spp_sweep_execute_n", self._handle, _lib.ptr(packed), nsamps, _lib.ptr(plane), ld,
+                  _lib.ptr(self.workspace), int(self.info.workspace_bytes), _lib.stream_handle(stream))
+        return plane
+
     def __del__(self) -> None:
         h = getattr(self, "_handle", None)
         if h is not None and h.value:

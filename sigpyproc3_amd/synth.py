@@ -159,3 +159,26 @@ def _torch():
 
         _t = torch
     return _t
+
+
+def cfg_batch_instance(i: int) -> tuple[SynthConfig, np.ndarray]:
+    """Instance ``i`` of the 64 uneven sweep instances (seed 3 + i).
+
+    nchans in {336, 1024, 4096}, nsamps ~ U[2^16, 2^19], nbits in {1, 2, 4, 8}, tsamp in
+    {64, 256} us, band 1250-1550 MHz (fch1 1550, foff -300/nchans), a DM list of
+    U[256, 4096] trials spaced linearly from 0 to a maximum DM ~ U[500, 3000].
+    Returns the configuration and the DM list.
+    """
+    rng = np.random.default_rng(3 + i)
+    nchans = int(rng.choice([336, 1024, 4096]))
+    nsamps = int(rng.integers(1 << 16, (1 << 19) + 1))
+    nbits = int(rng.choice([1, 2, 4, 8]))
+    tsamp = float(rng.choice([64e-6, 256e-6]))
+    ndm = int(rng.integers(256, 4097))
+    max_dm = float(rng.uniform(500.0, 3000.0))
+    dms = np.linspace(0.0, max_dm, ndm)
+    cfg = SynthConfig(nchans=nchans, nsamps=nsamps, nbits=nbits, fch1=1550.0, foff=-300.0 / nchans,
+                      tsamp=tsamp, seed=3 + i)
+    amp = 1.0 if nbits <= 2 else (3.0 if nbits == 4 else 12.0)
+    cfg.pulses = [Pulse(float(dms[ndm // 2]), nsamps // 4, 16, amp)]
+    return cfg, dms

@@ -215,6 +215,39 @@ def test_sweep_wide_accumulators(rng):
     _sweep_case(rng, 2100, 65600, 1, np.array([0.0, 1.0]), -0.004, 64e-6, variants=(1, 3))
 
 
+@pytest.mark.parametrize("nbits", [2, 8])
+def test_sweep_execute_n_shorter_gulps(nbits, rng):
+    """One plan (capacity nsamps) run on shorter prefixes, larger and smaller in turn, with
+    the same plane buffer: every execution equals the oracle on that prefix."""
+    from sigpyproc3_amd import _lib
+    from sigpyproc3_amd.sweep import SweepPlan
+
+    nsamps, nchans = 24000, 256
+    order = O.DEFAULT_BITORDER.get(nbits, "big")
+    samples = rng.integers(0, 1 << nbits, nsamps * nchans, dtype=np.uint8)
+    packed = samples if nbits == 8 else O.pack(samples, nbits, order)
+    f = O.chan_freqs(nchans, -1.0, 1500.0)
+    table = np.atleast_2d(O.compute_dmdelays(f, np.arange(0, 40, 0.5), 64e-6, 1500.0)).astype(np.int32)
+    plan = SweepPlan(table, nsamps, nbits, 1 if order == "big" else 0)
+    dev = _lib.to_device(packed)
+    plane = _lib.empty((plan.ndm * plan.max_len,), "float32")
+    md = int(table.max())
+    for n in (nsamps, md + 1, 9999, nsamps - 1, md + 3000, 2 * md + 17):
+        ld = int(plan.row_len_for(n).max())
+        plan.execute_n(dev, n, plane, ld=ld)
+        got = plane[: plan.ndm * ld].reshape(plan.ndm, ld).cpu().numpy()
+        ref = c_oracle.sweep(samples[: n * nchans], nchans, table, ld=ld)
+        rl = plan.row_len_for(n)
+        for i in range(plan.ndm):
+            np.testing.assert_array_equal(got[i, : rl[i]], ref[i, : rl[i]], err_msg=f"n {n} row {i}")
+            assert not got[i, rl[i]:].any()
+    with pytest.raises(ValueError, match="exceed"):
+        plan.execute_n(dev, md, plane)
+    bigger = _lib.zeros(((nsamps + 64) * nchans * nbits // 8,), "uint8")
+    with pytest.raises(ValueError, match="exceeds"):
+        plan.execute_n(bigger, nsamps + 64, plane, ld=nsamps)
+
+
 def test_sweep_rows_equal_filterbank_dedisperse(filfiles):
     from sigpyproc3_amd import FilReader
 
