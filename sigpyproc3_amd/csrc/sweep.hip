@@ -10,21 +10,24 @@
 //
 // Stage 2 (k_sweep): a block owns a DM tile (<= 16 consecutive DM rows whose
 // per-channel delay spread is <= SW_SPREAD_MAX, formed by the host planner) and IL*256
-// output samples. Channels are processed in stages of CC: the stage's windows
-// G[c][jb .. jb + 256 + spread] (jb = t0 + min delay of the tile, rounded down to 4 words)
-// are prefetched into registers while the previous stage is summed, then stored to the
-// other half of a double-buffered LDS area (one barrier per stage). Each wave owns K DM
-// rows; per channel and DM row a lane reads 4 LDS words (lanes read consecutive words:
-// bank-conflict free) and adds them into SWAR accumulators:
-//   nibble fields: 2 channels per v_add3, widened to byte lanes every GN channels
-//                  (nibbles stay <= 15), then to 16-bit lanes before bytes can overflow;
-//   byte fields (4-bit): widened to 16-bit lanes every stage;
-//   8-bit data: even/odd bytes go straight to 16-bit lanes;
+// output samples. Channels are processed in stages of CC: wave w stages channels
+// w, w + WAVES, ... of the next stage, reading the windows G[c][jb .. jb + 256 + spread]
+// (jb = t0 + min delay of the tile, rounded down to 4 words) with raw buffer loads whose
+// range is the channel's valid words (past them the hardware returns 0), into registers
+// while the current stage is summed, then into the other half of a double-buffered LDS
+// area (one barrier per stage). Each wave owns K DM rows; per channel and DM row a lane
+// reads 4 LDS words (lanes read consecutive words: bank-conflict free) at a byte offset
+// the planner's relative delays give, and adds them into SWAR accumulators:
+//   nibble fields (1/2-bit): groups of GN channels (GN * maxval <= 15: 5 for 2-bit) summed
+//                  with v_add3, two groups widened to byte lanes together, byte lanes to
+//                  16-bit lanes every SB stages;
+//   byte fields (4-bit): summed over the stage, then widened to 16-bit lanes;
+//   8-bit data: even/odd bytes (v_and / v_perm) go straight to 16-bit lanes;
 //   32-bit lanes only when nchans * maxval > 65535.
 // Integer sums are exact, so the float32 output is bit-exact to the reference's float64 sums.
 //
-// Work: nchans adds per DM-trial sample. LDS bytes read per sample-add: 4/IL... i.e. 0.5 for
-// nibble fields, 1 for byte fields. The sweep is bound by LDS reads and VALU, not HBM.
+// Work: nchans adds per DM-trial sample. LDS bytes read per sample-add: 4/IL (0.5 for
+// nibble fields, 1 for byte fields). The sweep is bound by LDS reads and VALU issue, not HBM.
 #include "spp_common.hpp"
 
 #include <algorithm>
@@ -323,7 +326,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
             }
         };
         if constexpr (IL == 8) {
-#pragma unroll 1
+#pragma unroll
             for (int g = 0; g < NG; g += 2)
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
@@ -512,7 +515,7 @@ extern "C" int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int
         i = j;
     }
     p->use32 = static_cast<int64_t>(nchans) * maxv > 65535;
-    p->variant = 1;  // 8 waves x 2 DM rows, 16 channels per stage (tunable per plan)
+    p->variant = (nbits <= 2) ? 2 : 1;  // 8 waves x 2 DM rows x 16 channels (tunable per plan)
     p->gpitch = ((nsamps + 63) / 64) * 64;
     const int64_t ntiles = static_cast<int64_t>(p->tile_dm0.size());
     int64_t off = 0;
