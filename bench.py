@@ -245,6 +245,69 @@ def bench_mf(args) -> int:
     return 0
 
 
+def bench_file(args) -> int:
+    """The 2-bit 4096-ch 2^20-sample stream as a SIGPROC file (1 GiB of data): one-DM
+    Filterbank.dedisperse through the double-buffered gulp pipeline and through a single
+    caller buffer, and Filterbank.dedisperse_sweep over DM 0..2047 straight from the file."""
+    import tempfile
+
+    import torch
+
+    from sigpyproc3_amd import FilReader, _lib
+    from sigpyproc3_amd.synth import cfg_sweep_2bit, generate_packed, write_sigproc
+
+    _lib.load()
+    _lib.require_gpu()
+    cfg = cfg_sweep_2bit()
+    with tempfile.TemporaryDirectory() as tmp:
+        path = f"{tmp}/cfg2.fil"
+        write_sigproc(path, cfg, generate_packed(cfg, device=_lib.device()))
+        torch.cuda.synchronize()
+        fil = FilReader(path)
+        nbytes = fil.header.nsamples * fil.samp_stride
+        with open(path, "rb") as fp:        # page-cache warm, as a re-read of a recent file
+            while fp.read(1 << 26):
+                pass
+        dm = 1000.0
+        runs = {}
+        for name, kw in (("pipeline", {}), ("single_buffer", {"allocator": bytearray})):
+            fil.dedisperse(dm, gulp=65536, **kw)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            tim = fil.dedisperse(dm, gulp=65536, **kw)
+            runs[name] = time.perf_counter() - t0
+            runs[name + "_first_value"] = float(tim.data[0])
+        dms = np.arange(2048, dtype=np.float64)
+        fil.dedisperse_sweep(dms[:16])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = fil.dedisperse_sweep(dms)
+        torch.cuda.synchronize()
+        t_sweep = time.perf_counter() - t0
+        total = int(res.row_len.sum())
+        del res
+    result = {
+        "metric": "file-resident workloads: one-DM dedisperse and the 2048-DM sweep from a SIGPROC file",
+        "value": total / t_sweep,
+        "unit": UNIT,
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic SIGPROC file in the page cache",
+        "config": {"workload": "2-bit, 4096 ch x 2^20 samples file (1 GiB of data)", "dm": dm, "gulp": 65536},
+        "dedisperse_s": {"pipeline": runs["pipeline"], "single_buffer": runs["single_buffer"]},
+        "dedisperse_file_gbs": {"pipeline": nbytes / runs["pipeline"] / 1e9,
+                                "single_buffer": nbytes / runs["single_buffer"] / 1e9},
+        "sweep_from_file_s": t_sweep,
+        "outputs_equal": runs["pipeline_first_value"] == runs["single_buffer_first_value"],
+    }
+    log(f"file: dedisperse {runs['pipeline']:.3f} s (pipeline) / {runs['single_buffer']:.3f} s (single "
+        f"buffer); sweep from file {t_sweep:.3f} s")
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -258,10 +321,11 @@ def main() -> int:
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="first run a plain device copy of this many GiB three times (the "
                          "reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--workload", choices=("sweep", "serve", "mf"), default="sweep",
+    ap.add_argument("--workload", choices=("sweep", "serve", "mf", "file"), default="sweep",
                     help="sweep: the 2-bit 2048-DM sweep (the headline line); serve: the "
                          "16-stream gulp service with p50/p99 latency; mf: the 8-bit 1024-DM "
-                         "sweep + normalisation + boxcar matched filter")
+                         "sweep + normalisation + boxcar matched filter; file: the same stream "
+                         "read from a SIGPROC file")
     ap.add_argument("--serve-gulps", type=int, default=160, help="gulps per serve run")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -273,6 +337,8 @@ def main() -> int:
         return bench_serve(args)
     if args.workload == "mf":
         return bench_mf(args)
+    if args.workload == "file":
+        return bench_file(args)
 
     import torch
 

@@ -130,6 +130,37 @@ int spp_sweep_execute_n(const spp_sweep_plan* plan, const uint8_t* packed, int64
                         void* stream);
 
 /*
+ * Gulp pre-steps (same reads and unpack as spp_dedisperse; `in` is a gulp in file order,
+ * packed nbits 1/2/4/8 with bit order `big`, or float32 for nbits == 32).
+ *
+ * spp_bandpass: per-channel sums of one gulp added into out[nchans] (device, float32).
+ * parallel == 0 replaces kernels.extract_bpass (core/kernels.py:277-291): float32 adds in
+ * time order per channel. parallel != 0 replaces kernels.extract_bpass_par
+ * (core/kernels.py:294-310): out[c] += float32(float64 sum over the gulp). Called by
+ * Filterbank.bandpass (base.py:308-358).
+ */
+int spp_bandpass(const void* in, int nbits, int big, int64_t nchans, int64_t nsamps, float* out,
+                 int parallel, void* stream);
+/*
+ * spp_mask_bytes: in place, byte j of every row_bytes-long row becomes
+ * (b & keep[j]) | set[j] (keep/set: device uint8 [row_bytes]). With keep/set built from a
+ * channel mask and value this is kernels.mask_channels (core/kernels.py:321-345) followed by
+ * the reference's pack, applied to the file's own bytes. Called by
+ * Filterbank.apply_channel_mask (base.py:530-583).
+ */
+int spp_mask_bytes(uint8_t* data, int64_t nbytes, int64_t row_bytes, const uint8_t* keep,
+                   const uint8_t* set, void* stream);
+/*
+ * spp_remove_zerodm: kernels.remove_zerodm (core/kernels.py:469-500) as called by
+ * Filterbank.remove_zerodm (base.py:961-1028). For each sample t, z = sum_c x[t][c] and
+ * out[t][c] = fma(-z, chanwts[c], x[t][c]) + bpass[c] in float64, stored as float32
+ * (nbits == 32) or as the low 8 bits of its truncation (one byte per sample, unpacked).
+ * chanwts, bpass: device float32 [nchans].
+ */
+int spp_remove_zerodm(const void* in, int nbits, int big, int64_t nchans, int64_t nsamps,
+                      const float* chanwts, const float* bpass, void* out, void* stream);
+
+/*
  * Running normalisation of DM series (no reference counterpart: the running mean/std
  * pre-step before MatchedFilter). For each row i of length row_len_host[i]:
  *   z[t] = (x[t] - m[t]) / s[t], m/s = mean/std over x padded symmetrically as

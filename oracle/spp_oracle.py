@@ -70,11 +70,11 @@ def unpack(array: np.ndarray, nbits: int, bitorder: str = "big") -> np.ndarray:
 
 
 def pack(array: np.ndarray, nbits: int, bitorder: str = "big") -> np.ndarray:
-    """kernels.pack{1,2,4}_8_{big,little} (kernels.py:72-173): inverse of unpack."""
+    """kernels.pack{1,2,4}_8_{big,little} (kernels.py:72-173): inverse of unpack. Values are
+    shifted and OR-ed without masking, so out-of-range samples spill into higher fields."""
     array = np.asarray(array, dtype=np.uint8)
     nper = 8 // nbits
-    mask = (1 << nbits) - 1
-    a = array.reshape(-1, nper).astype(np.uint32) & mask
+    a = array.reshape(-1, nper).astype(np.uint32)
     out = np.zeros(a.shape[0], dtype=np.uint32)
     for k in range(nper):
         shift = (8 - (k + 1) * nbits) if bitorder[0] == "b" else k * nbits
@@ -285,6 +285,95 @@ def filterbank_subband(stream, nsamples, nchans, delays, nsub, gulp=16384, start
         pieces.append(out_ar[: (nsamps_r - max_delay) * nsub].copy())
         pos += nsamps_r + skip
     return np.concatenate(pieces)
+
+
+# ---------------------------------------------------------------------------
+# gulp pre-steps: collapse, bandpass, channel mask, zero-DM (kernels.py:255-345, 465-500)
+# ---------------------------------------------------------------------------
+
+
+def collapse(stream, nsamples: int, nchans: int, gulp=16384, start=0, nsamps=None, method="sum"):
+    """Filterbank.collapse (base.py:256-306): kernels.extract_tim (kernels.py:255-274) per
+    gulp, written at ``ii * nsamps_r`` as the reference does, then ``/= nchans`` for mean."""
+    tim_len = (nsamples - start) if nsamps is None else nsamps
+    tim = np.zeros(tim_len, dtype=np.float32)
+    x = np.asarray(stream).reshape(-1, nchans)
+    pos = start
+    for ii, nsamps_r, skip in read_plan_blocks(tim_len, gulp, 0):
+        rows = x[pos : pos + nsamps_r].astype(np.float64)
+        tim[ii * nsamps_r : ii * nsamps_r + nsamps_r] = rows.sum(axis=1).astype(np.float32)
+        pos += nsamps_r + skip
+    if method == "mean":
+        tim /= nchans
+    return tim
+
+
+def bandpass(stream, nsamples: int, nchans: int, gulp=16384, start=0, nsamps=None,
+             parallel=False, method="sum"):
+    """Filterbank.bandpass (base.py:308-358).
+
+    parallel=False: kernels.extract_bpass (kernels.py:277-291), ``out[c] += x[t][c]`` in
+    float32 in time order. parallel=True: kernels.extract_bpass_par (kernels.py:294-310),
+    ``out[c] += float32(sum_t float64(x[t][c]))`` per gulp.
+    """
+    n = (nsamples - start) if nsamps is None else nsamps
+    out = np.zeros(nchans, dtype=np.float32)
+    x = np.asarray(stream).reshape(-1, nchans)
+    pos = start
+    num = 0
+    for _ii, nsamps_r, skip in read_plan_blocks(n, gulp, 0):
+        rows = x[pos : pos + nsamps_r]
+        if parallel:
+            out += rows.astype(np.float64).sum(axis=0).astype(np.float32)
+        else:
+            for t in range(nsamps_r):
+                out += rows[t].astype(np.float32)
+        num += nsamps_r
+        pos += nsamps_r + skip
+    if method == "mean":
+        out /= num
+    return out
+
+
+def mask_channels(stream, nchans: int, chan_mask, mask_value):
+    """kernels.mask_channels (kernels.py:321-345): masked channels set to ``mask_value``
+    (already cast to the data dtype), on a copy of the unpacked stream."""
+    x = np.array(stream).reshape(-1, nchans)
+    x[:, np.asarray(chan_mask, dtype=bool)[:nchans]] = mask_value
+    return x.reshape(-1)
+
+
+def fma_exact(a: float, b: float, c: float) -> float:
+    """a*b + c rounded once to float64 (exact rational arithmetic)."""
+    from fractions import Fraction
+
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def remove_zerodm(stream, nchans: int, bpass, out_dtype, fused=True):
+    """kernels.remove_zerodm (kernels.py:469-500) on the unpacked stream.
+
+    ``z = sum_c x[t][c]`` (float64), ``y = x - z*w + b`` with ``w = bpass / bpass.sum()``
+    (float32, numpy), evaluated in float64. fused=True forms ``x - z*w`` with one rounding
+    (the contraction fastmath allows; exact, slow: small inputs only), fused=False with two.
+    uint8 output keeps the low 8 bits of the truncated value (the float64 -> uint8 store on
+    x86-64); float32 output rounds once.
+    """
+    bpass = np.asarray(bpass, dtype=np.float32)
+    w = bpass / bpass.sum()
+    x = np.asarray(stream).reshape(-1, nchans).astype(np.float64)
+    z = x.sum(axis=1)
+    w64, b64 = w.astype(np.float64), bpass.astype(np.float64)
+    if fused:
+        y = np.empty_like(x)
+        for t in range(x.shape[0]):
+            for c in range(nchans):
+                y[t, c] = fma_exact(-z[t], float(w64[c]), float(x[t, c])) + b64[c]
+    else:
+        y = (x - z[:, None] * w64[None, :]) + b64[None, :]
+    if np.dtype(out_dtype) == np.float32:
+        return y.astype(np.float32).reshape(-1)
+    return (np.trunc(y).astype(np.int64) & 0xFF).astype(np.uint8).reshape(-1)
 
 
 # ---------------------------------------------------------------------------

@@ -58,6 +58,75 @@ class _GulpUploader:
         return self.dev
 
 
+class _GulpPipeline:
+    """Packed gulps into HBM with the file read of the next gulp overlapping the current one.
+
+    Two pinned host buffers and two device buffers alternate. A reader thread runs
+    ``read_plan_packed_into`` (file reads release the GIL) and reads gulp k + 1 while gulp k
+    is copied to the device and its kernels run. A host buffer is handed back to the reader
+    once its copy has completed; a device buffer is rewritten only by a copy enqueued on the
+    same stream after the kernels that read it.
+    """
+
+    def __init__(self, reader, gulp: int, start: int, nsamps: int | None, skipback: int) -> None:
+        import queue
+        import threading
+
+        t = _lib.torch()
+        nsamps_eff = (reader.header.nsamples - start) if nsamps is None else nsamps
+        nbytes = max(1, min(nsamps_eff, gulp) * reader.samp_stride)
+        self.host = [t.empty(nbytes, dtype=t.uint8, pin_memory=True) for _ in range(2)]
+        self.dev = [_lib.empty((nbytes,), "uint8") for _ in range(2)]
+        self.free = [threading.Semaphore(1), threading.Semaphore(1)]
+        self.items: queue.Queue = queue.Queue()  # at most 2 entries: one per acquired buffer
+        self._stop = False
+        bufs = [h.numpy() for h in self.host]
+
+        def produce() -> None:
+            try:
+                gen = reader.read_plan_packed_into(bufs, gulp=gulp, start=start, nsamps=nsamps,
+                                                   skipback=skipback)
+                k = 0
+                while True:
+                    self.free[k % 2].acquire()
+                    if self._stop:
+                        return
+                    item = next(gen, None)
+                    if item is None:
+                        break
+                    self.items.put(item)
+                    k += 1
+                self.items.put(None)
+            except BaseException as exc:  # noqa: BLE001 - re-raised in the consumer
+                self.items.put(exc)
+
+        self.thread = threading.Thread(target=produce, daemon=True)
+        self.thread.start()
+
+    def __iter__(self):
+        t = _lib.torch()
+        try:
+            while True:
+                item = self.items.get()
+                if item is None:
+                    return
+                if isinstance(item, BaseException):
+                    raise item
+                nsamps_r, ii, i, nbytes = item
+                dev = self.dev[i]
+                dev[:nbytes].copy_(self.host[i][:nbytes], non_blocking=True)
+                ev = t.cuda.Event()
+                ev.record()
+                yield nsamps_r, ii, dev, nbytes
+                ev.synchronize()          # the copy has read host buffer i
+                self.free[i].release()
+        finally:
+            self._stop = True
+            for f in self.free:
+                f.release()
+            self.thread.join()
+
+
 class Filterbank(ABC):
     """Base class of frequency-major filterbank data readers (base.py:36-60)."""
 
@@ -106,15 +175,7 @@ class Filterbank(ABC):
         tim = _lib.zeros((max(tim_len, 1),), "float32")
         d_dev = _lib.to_device(np.ascontiguousarray(chan_delays, dtype=np.int32))
         stream = _lib.stream_handle()
-        nsamps_eff = (hdr.nsamples - start) if nsamps is None else nsamps
-        gulp_eff = min(nsamps_eff, gulp)
-        up = _GulpUploader(max(1, gulp_eff * self.samp_stride))
-        plan_kwargs.pop("description", None)
-        plan_kwargs.pop("quiet", None)
-        allocator = plan_kwargs.pop("allocator", None) or up.allocator()
-        for nsamps_r, ii, raw in self.read_plan_packed(
-                gulp=gulp, start=start, nsamps=nsamps, skipback=max_delay, allocator=allocator):
-            dev = up.upload(raw)
+        for nsamps_r, ii, dev, _nb in self._device_gulps(gulp, start, nsamps, plan_kwargs, max_delay):
             _lib.call("spp_dedisperse", _lib.ptr(dev), nbits, big, hdr.nchans, nsamps_r,
                       _lib.ptr(d_dev), max_delay, _lib.ptr(tim), ii * (gulp - max_delay), stream)
         data = tim[:tim_len].cpu().numpy() if tim_len > 0 else np.zeros(0, dtype=np.float32)
@@ -147,20 +208,180 @@ class Filterbank(ABC):
         cts_dev = _lib.to_device(chan_to_sub)
         out_dev = _lib.zeros((max(out_len, 1),), "float32")
         stream = _lib.stream_handle()
-        nsamps_eff = (hdr.nsamples - start) if nsamps is None else nsamps
-        up = _GulpUploader(max(1, min(nsamps_eff, gulp) * self.samp_stride))
-        for k in ("description", "quiet"):
-            plan_kwargs.pop(k, None)
-        allocator = plan_kwargs.pop("allocator", None) or up.allocator()
         try:
-            for nsamps_r, _ii, raw in self.read_plan_packed(
-                    gulp=gulp, start=start, nsamps=nsamps, skipback=max_delay, allocator=allocator):
-                dev = up.upload(raw)
+            for nsamps_r, _ii, dev, _nb in self._device_gulps(gulp, start, nsamps, plan_kwargs, max_delay):
                 _lib.call("spp_subband", _lib.ptr(dev), nbits, big, hdr.nchans, nsamps_r,
                           _lib.ptr(d_dev), _lib.ptr(cts_dev), max_delay, nsub, _lib.ptr(out_dev),
                           out_len, stream)
                 nwrite = (nsamps_r - max_delay) * nsub
                 out_file.cwrite(out_dev[:nwrite].cpu().numpy())
+        finally:
+            out_file.close()
+        return outfile_name
+
+    def _device_gulps(self, gulp: int, start: int, nsamps: int | None, plan_kwargs: dict,
+                      skipback: int = 0):
+        """(nsamps_r, ii, device bytes, nbytes) of every packed gulp of the read_plan schedule.
+
+        Without a caller ``allocator`` the gulps go through ``_GulpPipeline`` (double-buffered
+        pinned memory, reads overlapping the device work); with one, every gulp is read into
+        the caller's buffer and copied from there.
+        """
+        for k in ("description", "quiet"):
+            plan_kwargs.pop(k, None)
+        allocator = plan_kwargs.pop("allocator", None)
+        if allocator is None and hasattr(self, "read_plan_packed_into"):
+            yield from _GulpPipeline(self, gulp, start, nsamps, skipback)
+            return
+        nsamps_eff = (self.header.nsamples - start) if nsamps is None else nsamps
+        up = _GulpUploader(max(1, min(nsamps_eff, gulp) * self.samp_stride))
+        for nsamps_r, ii, raw in self.read_plan_packed(gulp=gulp, start=start, nsamps=nsamps,
+                                                       skipback=skipback,
+                                                       allocator=allocator or up.allocator()):
+            yield nsamps_r, ii, up.upload(raw), raw.size
+
+    def collapse(self, method: str = "sum", gulp: int = 16384, start: int = 0,
+                 nsamps: int | None = None, *, parallel: bool = False, **plan_kwargs) -> TimeSeries:
+        """Zero-DM time series: the sum over channels per sample (base.py:256-306).
+
+        ``kernels.extract_tim`` is ``dedisperse`` with every delay 0, so this runs the same
+        fused kernel (``spp_dedisperse``) with a zero delay table. Gulp ``ii`` is written at
+        offset ``ii * nsamps_r``, as the reference does (base.py:297-303).
+        """
+        del parallel
+        hdr = self.header
+        tim_len = (hdr.nsamples - start) if nsamps is None else nsamps
+        nbits, big = self._kernel_nbits()
+        _lib.require_gpu()
+        tim = _lib.zeros((max(tim_len, 1),), "float32")
+        zero = _lib.zeros((hdr.nchans,), "int32")
+        stream = _lib.stream_handle()
+        for nsamps_r, ii, dev, _nb in self._device_gulps(gulp, start, nsamps, plan_kwargs):
+            if ii * nsamps_r + nsamps_r > tim_len:
+                msg = "output offset beyond the time series"
+                raise ValueError(msg)
+            _lib.call("spp_dedisperse", _lib.ptr(dev), nbits, big, hdr.nchans, nsamps_r,
+                      _lib.ptr(zero), 0, _lib.ptr(tim), ii * nsamps_r, stream)
+        tim_ar = tim[:tim_len].cpu().numpy() if tim_len > 0 else np.zeros(0, dtype=np.float32)
+        if method == "mean":
+            tim_ar /= hdr.nchans
+        return TimeSeries(tim_ar, hdr.new_header({"nchans": 1, "dm": 0}))
+
+    def bandpass(self, method: str = "sum", gulp: int = 16384, start: int = 0, *,
+                 parallel: bool = False, nsamps: int | None = None, **plan_kwargs) -> TimeSeries:
+        """Per-channel sums over time (base.py:308-358) on the GPU (``spp_bandpass``).
+
+        ``parallel=False`` accumulates in float32 in time order per channel, as
+        ``kernels.extract_bpass`` does; ``parallel=True`` adds each gulp's float64 sum once,
+        as ``kernels.extract_bpass_par`` does.
+        """
+        hdr = self.header
+        nbits, big = self._kernel_nbits()
+        _lib.require_gpu()
+        bpass = _lib.zeros((hdr.nchans,), "float32")
+        stream = _lib.stream_handle()
+        num_samples = 0
+        for nsamps_r, _ii, dev, _nb in self._device_gulps(gulp, start, nsamps, plan_kwargs):
+            _lib.call("spp_bandpass", _lib.ptr(dev), nbits, big, hdr.nchans, nsamps_r,
+                      _lib.ptr(bpass), 1 if parallel else 0, stream)
+            num_samples += nsamps_r
+        bpass_ar = bpass.cpu().numpy()
+        if method == "mean":
+            bpass_ar /= num_samples
+        return TimeSeries(bpass_ar, hdr.new_header({"nchans": 1, "nsamples": len(bpass_ar)}))
+
+    def _mask_tables(self, chan_mask: np.ndarray, mask_value) -> tuple[np.ndarray, np.ndarray]:
+        """keep/set byte tables of one row such that (b & keep) | set equals masking the
+        unpacked samples and packing them again the reference's way (no field masking)."""
+        hdr = self.header
+        nbits = hdr.nbits
+        nchans = hdr.nchans
+        if nbits == 32:
+            keep = np.full(nchans * 4, 0xFF, dtype=np.uint8)
+            setb = np.zeros(nchans * 4, dtype=np.uint8)
+            vb = np.frombuffer(np.float32(mask_value).tobytes(), dtype=np.uint8)
+            for c in np.flatnonzero(chan_mask[:nchans]):
+                keep[4 * c : 4 * c + 4] = 0
+                setb[4 * c : 4 * c + 4] = vb
+            return keep, setb
+        if nbits == 8:
+            m = chan_mask[:nchans]
+            keep = np.where(m, 0, 0xFF).astype(np.uint8)
+            setb = np.where(m, int(mask_value), 0).astype(np.uint8)
+            return keep, setb
+        per = 8 // nbits
+        big = nbits in (2, 4)
+        fmask = (1 << nbits) - 1
+        keep = np.zeros(nchans // per, dtype=np.uint32)
+        setb = np.zeros(nchans // per, dtype=np.uint32)
+        v = int(mask_value)
+        for k in range(per):
+            shift = (8 - (k + 1) * nbits) if big else k * nbits
+            m = chan_mask[k:nchans:per]
+            keep |= np.where(m, 0, fmask << shift).astype(np.uint32)
+            setb |= np.where(m, v << shift, 0).astype(np.uint32)
+        return (keep & 0xFF).astype(np.uint8), (setb & 0xFF).astype(np.uint8)
+
+    def apply_channel_mask(self, chan_mask, mask_value: float = 0, outfile_name: str | None = None,
+                           gulp: int = 16384, start: int = 0, nsamps: int | None = None, *,
+                           parallel: bool = False, **plan_kwargs) -> str:
+        """Replace masked channels and write a new file (base.py:530-583).
+
+        The mask is applied to the file's own bytes on the GPU (``spp_mask_bytes``); the
+        result equals masking the unpacked samples and packing them as the reference does.
+        """
+        del parallel
+        hdr = self.header
+        if outfile_name is None:
+            outfile_name = f"{hdr.basename}_masked.fil"
+        chan_mask = np.array(chan_mask).astype("bool")
+        if chan_mask.size < hdr.nchans:
+            msg = f"chan_mask has {chan_mask.size} entries < nchans = {hdr.nchans}"
+            raise ValueError(msg)
+        self._kernel_nbits()
+        mask_value = np.float32(mask_value).astype(hdr.dtype)
+        keep, setb = self._mask_tables(chan_mask, mask_value)
+        _lib.require_gpu()
+        keep_d, set_d = _lib.to_device(keep), _lib.to_device(setb)
+        stream = _lib.stream_handle()
+        out_file = hdr.prep_outfile(outfile_name)
+        try:
+            for _nr, _ii, dev, nb in self._device_gulps(gulp, start, nsamps, plan_kwargs):
+                _lib.call("spp_mask_bytes", _lib.ptr(dev), nb, self.samp_stride, _lib.ptr(keep_d),
+                          _lib.ptr(set_d), stream)
+                out_file.write(dev[:nb].cpu().numpy().tobytes())
+        finally:
+            out_file.close()
+        return outfile_name
+
+    def remove_zerodm(self, outfile_name: str | None = None, gulp: int = 16384, start: int = 0,
+                      nsamps: int | None = None, *, parallel: bool = False, **plan_kwargs) -> str:
+        """Remove the channel-weighted zero-DM signal and write a new file (base.py:961-1028).
+
+        The bandpass is ``self.bandpass(**plan_kwargs)`` (whole file, float32 sums) and the
+        weights ``bpass / bpass.sum()`` in float32 on the host, as in the reference; each
+        gulp then runs ``spp_remove_zerodm`` on the GPU.
+        """
+        del parallel
+        hdr = self.header
+        if outfile_name is None:
+            outfile_name = f"{hdr.basename}_noZeroDM.fil"
+        nbits, big = self._kernel_nbits()
+        bpass = self.bandpass(**dict(plan_kwargs)).data
+        chanwts = bpass / bpass.sum()
+        _lib.require_gpu()
+        w_d = _lib.to_device(np.ascontiguousarray(chanwts, dtype=np.float32))
+        b_d = _lib.to_device(np.ascontiguousarray(bpass, dtype=np.float32))
+        out_dtype = "float32" if nbits == 32 else "uint8"
+        nsamps_eff = (hdr.nsamples - start) if nsamps is None else nsamps
+        out_dev = _lib.empty((max(1, min(nsamps_eff, gulp) * hdr.nchans),), out_dtype)
+        stream = _lib.stream_handle()
+        out_file = hdr.prep_outfile(outfile_name, nbits=hdr.nbits)
+        try:
+            for nsamps_r, _ii, dev, _nb in self._device_gulps(gulp, start, nsamps, plan_kwargs):
+                _lib.call("spp_remove_zerodm", _lib.ptr(dev), nbits, big, hdr.nchans, nsamps_r,
+                          _lib.ptr(w_d), _lib.ptr(b_d), _lib.ptr(out_dev), stream)
+                out_file.cwrite(out_dev[: nsamps_r * hdr.nchans].cpu().numpy())
         finally:
             out_file.close()
         return outfile_name

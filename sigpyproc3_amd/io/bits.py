@@ -79,8 +79,9 @@ def pack(array: np.ndarray, nbits: int, packed: np.ndarray | None = None, *,
     elif packed.size != array.size // bitfact:
         msg = f"packing array must be input size // {bitfact}, got {packed.size}"
         raise ValueError(msg)
-    mask = (1 << nbits) - 1
-    a = array[: packed.size * bitfact].reshape(-1, bitfact).astype(np.uint32) & mask
+    # Values are shifted and OR-ed without masking, as kernels.pack* do (core/kernels.py:72-173):
+    # an out-of-range sample spills into the neighbouring fields; bits above 7 are dropped.
+    a = array[: packed.size * bitfact].reshape(-1, bitfact).astype(np.uint32)
     acc = np.zeros(packed.size, dtype=np.uint32)
     for k in range(bitfact):
         shift = (8 - (k + 1) * nbits) if bitorder[0] == "b" else k * nbits

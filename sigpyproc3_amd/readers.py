@@ -88,6 +88,32 @@ class FilReader(Filterbank):
                                                 allocator=allocator, unpack=False):
             yield nsamps_r, ii, raw
 
+    def read_plan_packed_into(self, buffers, gulp: int = 16384, start: int = 0,
+                              nsamps: int | None = None, skipback: int = 0):
+        """The ``read_plan_packed`` schedule, reading gulp k into ``buffers[k % len(buffers)]``
+        (each at least ``gulp * samp_stride`` bytes). Yields ``(nsamps_read, ii, k % n, nbytes)``;
+        the caller must be done with a buffer before asking for the gulp that reuses it."""
+        if nsamps is None:
+            nsamps = self.header.nsamples - start
+        gulp, blocks = plan_blocks(nsamps, gulp, skipback)
+        views = [np.frombuffer(b, dtype=np.uint8) for b in buffers]
+        need = gulp * self.samp_stride
+        if any(v.size < need for v in views):
+            msg = f"read buffers must hold {need} bytes"
+            raise ValueError(msg)
+        self._file.seek(start * self.samp_stride)
+        nchans = self.header.nchans
+        for k, (ii, nrows, skip) in enumerate(blocks):
+            i = k % len(views)
+            nbytes = self._file.creadinto_packed(views[i][:need])
+            expected = int(nrows * nchans * self.chan_stride)
+            if nbytes != expected:
+                msg = f"Unexpected number of bytes read from file {nbytes} (actual) != {expected} (expected)"
+                raise ValueError(msg)
+            if skip != 0:
+                self._file.seek(int(skip * nchans * self.chan_stride), whence=1)
+            yield nrows, ii, i, expected
+
     def _gulps(self, gulp, start, nsamps, skipback, *, allocator, unpack: bool):
         if nsamps is None:
             nsamps = self.header.nsamples - start

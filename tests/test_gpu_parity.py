@@ -342,3 +342,162 @@ def test_matched_filter_detection_and_noise():
     rng = np.random.default_rng(42)
     noise = rng.normal(loc=5, scale=2, size=1000).astype(np.float32)
     assert MatchedFilter(noise).snr < 6
+
+
+# ---------------------------------------------------------------------------
+# gulp pre-steps: collapse, bandpass, channel mask, zero-DM removal
+# ---------------------------------------------------------------------------
+
+
+def _fixture(filfiles, key):
+    return filfiles[key] if not isinstance(filfiles[key], list) else filfiles[key]
+
+
+def _stream(path):
+    files = path if isinstance(path, list) else [path]
+    pieces = [O.read_sigproc(f) for f in files]
+    hdr = pieces[0][0]
+    return hdr, np.concatenate([O.unpacked_stream(h, r) for h, r in pieces]), sum(h["nsamples"] for h, _ in pieces)
+
+
+@pytest.mark.parametrize("key", ["1bit", "2bit", "4bit", "8bit", "tutorial"])
+def test_collapse_vs_oracle(filfiles, key):
+    from sigpyproc3_amd import FilReader
+
+    path = _fixture(filfiles, key)
+    hdr, s, n = _stream(path)
+    fil = FilReader(path)
+    for gulp, method in ((16384, "sum"), (1000, "sum"), (777, "mean")):
+        tim = fil.collapse(method=method, gulp=gulp)
+        np.testing.assert_array_equal(tim.data, O.collapse(s, n, hdr["nchans"], gulp=gulp, method=method))
+        assert tim.header.nchans == 1 and tim.header.dm == 0
+
+
+@pytest.mark.parametrize("key", ["1bit", "2bit", "4bit", "8bit", "tutorial"])
+def test_bandpass_vs_oracle(filfiles, key):
+    from sigpyproc3_amd import FilReader
+
+    path = _fixture(filfiles, key)
+    hdr, s, n = _stream(path)
+    fil = FilReader(path)
+    for parallel in (False, True):
+        for gulp, method in ((16384, "sum"), (999, "mean")):
+            bp = fil.bandpass(method=method, gulp=gulp, parallel=parallel)
+            ref = O.bandpass(s, n, hdr["nchans"], gulp=gulp, parallel=parallel, method=method)
+            np.testing.assert_array_equal(bp.data, ref)
+            assert bp.header.nsamples == hdr["nchans"]
+
+
+def test_bandpass_float_sequential_order(tmp_path, rng):
+    """float32 data: the sequential float32 accumulation order is reproduced exactly."""
+    from sigpyproc3_amd import FilReader
+    from sigpyproc3_amd.synth import SynthConfig
+
+    cfg = SynthConfig(nchans=40, nsamps=5000, nbits=32, fch1=1400.0, foff=-1.0, tsamp=1e-3, seed=0)
+    x = rng.normal(100.0, 30.0, (cfg.nsamps, cfg.nchans)).astype(np.float32)
+    path = tmp_path / "f32.fil"
+    from sigpyproc3_amd.io import sigproc
+
+    with open(path, "wb") as fp:
+        fp.write(sigproc.encode_header(cfg.header_dict()))
+        fp.write(x.tobytes())
+    bp = FilReader(str(path)).bandpass(gulp=1300)
+    np.testing.assert_array_equal(bp.data, O.bandpass(x.reshape(-1), cfg.nsamps, cfg.nchans, gulp=1300))
+
+
+@pytest.mark.parametrize("key", ["1bit", "2bit", "4bit", "8bit", "tutorial"])
+def test_apply_channel_mask_vs_oracle(filfiles, key, tmp_path, rng):
+    from sigpyproc3_amd import FilReader
+
+    path = _fixture(filfiles, key)
+    hdr, s, n = _stream(path)
+    fil = FilReader(path)
+    mask = rng.random(hdr["nchans"]) < 0.25
+    for value in (0, 3, 200):
+        out = fil.apply_channel_mask(mask, mask_value=value, outfile_name=str(tmp_path / f"m{value}.fil"),
+                                     gulp=1500)
+        h2, raw2 = O.read_sigproc(out)
+        v = np.float32(value).astype(fil.header.dtype)
+        masked = O.mask_channels(s, hdr["nchans"], mask, v)
+        expect = O.pack(masked, hdr["nbits"], O.DEFAULT_BITORDER[hdr["nbits"]]) if hdr["nbits"] < 8 else masked
+        np.testing.assert_array_equal(raw2, expect)
+        assert (h2["nchans"], h2["nbits"], h2["nsamples"]) == (hdr["nchans"], hdr["nbits"], n)
+
+
+def _zerodm_kernel(s_packed, nbits, big, nchans, nsamps, bpass, out_dtype):
+    from sigpyproc3_amd import _lib
+
+    w = bpass / bpass.sum()
+    dev = _lib.to_device(np.ascontiguousarray(s_packed))
+    out = _lib.empty((nsamps * nchans,), "float32" if out_dtype == np.float32 else "uint8")
+    w_d = _lib.to_device(w.astype(np.float32))       # keep the device copies alive
+    b_d = _lib.to_device(bpass.astype(np.float32))
+    _lib.call("spp_remove_zerodm", _lib.ptr(dev), nbits, big, nchans, nsamps, _lib.ptr(w_d),
+              _lib.ptr(b_d), _lib.ptr(out), _lib.stream_handle())
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("nbits", [2, 8, 32])
+def test_remove_zerodm_kernel_vs_exact_oracle(nbits, rng):
+    """Small inputs against the fused-multiply-add oracle evaluated in exact arithmetic."""
+    nchans, nsamps = 48, 300
+    if nbits == 32:
+        x = rng.normal(20.0, 5.0, (nsamps, nchans)).astype(np.float32)
+        packed, out_dtype = x.view(np.uint8), np.float32
+    else:
+        x = rng.integers(0, 1 << nbits, (nsamps, nchans), dtype=np.uint8)
+        packed = x if nbits == 8 else O.pack(x.reshape(-1), nbits, "big")
+        out_dtype = np.uint8
+    bpass = rng.uniform(1.0e3, 5.0e4, nchans).astype(np.float32)
+    got = _zerodm_kernel(packed, nbits, 1, nchans, nsamps, bpass, out_dtype)
+    np.testing.assert_array_equal(got, O.remove_zerodm(x.reshape(-1), nchans, bpass, out_dtype, fused=True))
+
+
+@pytest.mark.parametrize("key", ["2bit", "8bit", "tutorial"])
+def test_remove_zerodm_file_vs_oracle(filfiles, key, tmp_path, rng):
+    from sigpyproc3_amd import FilReader
+
+    path = _fixture(filfiles, key)
+    hdr, s, n = _stream(path)
+    fil = FilReader(path)
+    nch = hdr["nchans"]
+    out = fil.remove_zerodm(outfile_name=str(tmp_path / "z.fil"), gulp=2000)
+    bpass = O.bandpass(s, n, nch)        # Filterbank.bandpass() defaults, float32 sums
+    nbits = hdr["nbits"]
+    first = path[0] if isinstance(path, list) else path
+    raw = np.concatenate([O.read_sigproc(f)[1] for f in (path if isinstance(path, list) else [path])])
+    vals = _zerodm_kernel(raw, nbits, 1 if nbits in (2, 4) else 0, nch, n, bpass, np.uint8)
+    # the file holds the kernel's values packed the reference's way
+    h2, raw2 = O.read_sigproc(out)
+    expect = O.pack(vals, nbits, O.DEFAULT_BITORDER[nbits]) if nbits < 8 else vals
+    np.testing.assert_array_equal(raw2, expect)
+    assert (h2["nbits"], h2["nchans"]) == (nbits, nch) and first
+    # values: exact at sampled positions, within one count of the unfused form everywhere
+    x = s.reshape(-1, nch)
+    idx = rng.integers(0, x.shape[0], 200)
+    sub = O.remove_zerodm(x[idx].reshape(-1), nch, bpass, np.uint8, fused=True).reshape(-1, nch)
+    # z (the zero-DM) of each sampled row is computed from its own row, so rows can be sampled
+    np.testing.assert_array_equal(vals.reshape(-1, nch)[idx], sub)
+    loose = O.remove_zerodm(s, nch, bpass, np.uint8, fused=False)
+    diff = (vals.astype(np.int16) - loose.astype(np.int16)) % 256
+    bad = np.minimum(diff, 256 - diff)
+    assert bad.max() <= 1 and np.count_nonzero(bad) <= 1e-3 * bad.size
+
+
+def test_gulp_pipeline_matches_single_buffer_and_stops_early(filfiles):
+    """Double-buffered reads give the same results as the caller-allocator path, and a
+    consumer that stops early ends the reader thread."""
+    from sigpyproc3_amd import FilReader
+    from sigpyproc3_amd.base import _GulpPipeline
+
+    fil = FilReader(filfiles["tutorial"])
+    gulp = 3000
+    buf = bytearray(2 * gulp * fil.samp_stride)
+    a = fil.dedisperse(30, gulp=gulp)
+    b = fil.dedisperse(30, gulp=gulp, allocator=lambda n: buf[:n])
+    np.testing.assert_array_equal(a.data, b.data)
+    pipe = _GulpPipeline(fil, gulp, 0, None, 0)
+    it = iter(pipe)
+    next(it)
+    it.close()
+    assert not pipe.thread.is_alive()

@@ -239,3 +239,48 @@ def test_templates():
         Template.gen_boxcar(-1)
     with pytest.raises(ValueError):
         Template(np.zeros(10), 5, 20)
+
+
+@pytest.mark.parametrize("key", ["1bit", "2bit", "4bit", "8bit"])
+@pytest.mark.parametrize("value", [0, 1, 5, 300.7])
+def test_mask_tables_equal_unpack_mask_pack(filfiles, key, value, rng):
+    """(b & keep) | set on the file's bytes equals kernels.mask_channels on the unpacked
+    samples followed by the reference's pack (base.py:530-583)."""
+    from sigpyproc3_amd import FilReader
+
+    path = filfiles[key][0] if isinstance(filfiles[key], list) else filfiles[key]
+    fil = FilReader(path)
+    hdr = fil.header
+    h, raw = O.read_sigproc(path)
+    mask = rng.random(hdr.nchans) < 0.3
+    v = np.float32(value).astype(hdr.dtype)
+    keep, setb = fil._mask_tables(mask, v)
+    rows = raw[: 64 * fil.samp_stride].reshape(64, fil.samp_stride)
+    got = (rows & keep[None, :]) | setb[None, :]
+    x = O.unpacked_stream(h, raw)[: 64 * hdr.nchans]
+    masked = O.mask_channels(x, hdr.nchans, mask, v)
+    if hdr.nbits < 8:
+        order = O.DEFAULT_BITORDER[hdr.nbits]
+        expect = O.pack(masked, hdr.nbits, order)
+    else:
+        expect = masked.astype(np.uint8)
+    np.testing.assert_array_equal(got.reshape(-1), expect)
+
+
+@pytest.mark.parametrize("skipback", [0, 300])
+def test_read_plan_packed_into_alternates_buffers(filfiles, skipback):
+    """Same schedule and bytes as read_plan_packed, gulp k read into buffer k % 2."""
+    from sigpyproc3_amd import FilReader
+
+    fil = FilReader(filfiles["4bit"])
+    gulp = 1000
+    ref = [(n, ii, raw.copy()) for n, ii, raw in fil.read_plan_packed(gulp=gulp, skipback=skipback)]
+    bufs = [bytearray(gulp * fil.samp_stride), bytearray(gulp * fil.samp_stride)]
+    got = []
+    for k, (n, ii, i, nb) in enumerate(fil.read_plan_packed_into(bufs, gulp=gulp, skipback=skipback)):
+        assert i == k % 2
+        got.append((n, ii, np.frombuffer(bufs[i], dtype=np.uint8)[:nb].copy()))
+    assert len(got) == len(ref)
+    for (n1, i1, r1), (n2, i2, r2) in zip(ref, got):
+        assert (n1, i1) == (n2, i2)
+        np.testing.assert_array_equal(r1, r2)

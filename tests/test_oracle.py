@@ -296,3 +296,77 @@ def test_running_normalise_brute_force(rng):
         sd = win.std()
         ref = (x[t] - win.mean()) / (sd if sd > 0 else 1.0)
         assert abs(float(z[t]) - ref) <= 1e-5 * max(1.0, abs(ref))
+
+
+# ---------------------------------------------------------------------------
+# gulp pre-steps (kernels.py:255-345, 465-500)
+# ---------------------------------------------------------------------------
+
+
+def test_pack_spills_out_of_range_values():
+    # kernels.pack2_8_big / pack4_8_little / pack1_8_big OR shifted values unmasked
+    # (kernels.py:72-173); bits above 7 are dropped by the uint8 store.
+    a = np.array([0, 5, 0, 0, 7, 0, 0, 1], dtype=np.uint8)
+    assert O.pack(a, 2, "big").tolist() == [(5 << 4) & 0xFF, ((7 << 6) | 1) & 0xFF]
+    b = np.array([17, 2], dtype=np.uint8)
+    assert O.pack(b, 4, "little").tolist() == [(17 | (2 << 4)) & 0xFF]
+    c = np.array([2, 0, 0, 0, 0, 0, 0, 3], dtype=np.uint8)
+    assert O.pack(c, 1, "big").tolist() == [((2 << 7) | 3) & 0xFF]
+
+
+def test_collapse_offsets_and_sums(rng):
+    x = rng.integers(0, 256, (10, 3), dtype=np.uint8)
+    # gulp 4 over 10 samples: gulps (0, 4), (1, 4), (2, 2); the last lands at 2 * 2 = 4
+    expect = x.sum(axis=1).astype(np.float32)
+    got = O.collapse(x.reshape(-1), 10, 3, gulp=4)
+    exp = np.zeros(10, dtype=np.float32)
+    exp[0:4], exp[4:8] = expect[0:4], expect[4:8]
+    exp[4:6] = expect[8:10]
+    np.testing.assert_array_equal(got, exp)
+    # a dividing gulp gives the plain channel sum; mean divides by nchans in float32
+    np.testing.assert_array_equal(O.collapse(x.reshape(-1), 10, 3, gulp=5), expect)
+    np.testing.assert_array_equal(O.collapse(x.reshape(-1), 10, 3, gulp=5, method="mean"),
+                                  expect / np.float32(3))
+
+
+def test_bandpass_sequential_float32_and_exact_integers(rng):
+    x = rng.normal(1000.0, 300.0, (257, 5)).astype(np.float32)
+    got = O.bandpass(x.reshape(-1), 257, 5, gulp=100)
+    for c in range(5):                       # brute force: scalar float32 adds in time order
+        acc = np.float32(0.0)
+        for t in range(257):
+            acc = np.float32(acc + x[t, c])
+        assert got[c] == acc
+    u = rng.integers(0, 256, (3000, 7), dtype=np.uint8)
+    exact = u.astype(np.int64).sum(axis=0)   # < 2^24: every order gives the integer
+    np.testing.assert_array_equal(O.bandpass(u.reshape(-1), 3000, 7, gulp=700), exact.astype(np.float32))
+    np.testing.assert_array_equal(O.bandpass(u.reshape(-1), 3000, 7, gulp=700, parallel=True),
+                                  exact.astype(np.float32))
+    np.testing.assert_array_equal(O.bandpass(u.reshape(-1), 3000, 7, gulp=700, method="mean"),
+                                  exact.astype(np.float32) / np.float32(3000))
+
+
+def test_fma_exact_single_rounding():
+    a = 1.0 + 2.0**-30
+    # a*a = 1 + 2^-29 + 2^-60: two roundings lose the 2^-60 term, one rounding keeps it
+    assert O.fma_exact(a, a, -1.0) == 2.0**-29 + 2.0**-60
+    assert a * a - 1.0 == 2.0**-29
+
+
+def test_remove_zerodm_identities(rng):
+    nchans = 16
+    x = rng.integers(0, 256, (40, nchans), dtype=np.uint8)
+    # flat bandpass: w = 1/16 exactly, so fused and unfused agree and y = x - mean + b exactly
+    b = np.full(nchans, 100.0, dtype=np.float32)
+    y32 = O.remove_zerodm(x.reshape(-1), nchans, b, np.float32).reshape(-1, nchans)
+    expect = x - x.astype(np.float64).mean(axis=1, keepdims=True) + 100.0
+    np.testing.assert_array_equal(y32, expect.astype(np.float32))
+    np.testing.assert_array_equal(y32, O.remove_zerodm(x.reshape(-1), nchans, b, np.float32,
+                                                       fused=False).reshape(-1, nchans))
+    # uint8 output: low 8 bits of the truncation toward zero
+    u8 = O.remove_zerodm(x.reshape(-1), nchans, b, np.uint8).reshape(-1, nchans)
+    np.testing.assert_array_equal(u8, (np.trunc(expect).astype(np.int64) & 0xFF).astype(np.uint8))
+    # weighted bandpass: the weighted zero-DM is removed, sum_c (y - b) = 0 up to rounding
+    b2 = rng.uniform(50, 150, nchans).astype(np.float32)
+    y = O.remove_zerodm(x.reshape(-1), nchans, b2, np.float32).reshape(-1, nchans).astype(np.float64)
+    np.testing.assert_allclose((y - b2).sum(axis=1), 0.0, atol=1e-3)
