@@ -161,6 +161,24 @@ int spp_remove_zerodm(const void* in, int nbits, int big, int64_t nchans, int64_
                       const float* chanwts, const float* bpass, void* out, void* stream);
 
 /*
+ * In-memory blocks: float32 [nrows][ncols] (channel-major FilterbankBlock data). Row r is
+ * read at column (t + off[r]) mod ncols; off_host and off hold the same int64 offsets (host
+ * copy for validation, device copy for the kernel), each in [0, ncols). out_cols < ncols is
+ * the valid-columns form, where no offset may wrap.
+ *
+ * spp_roll_rows: out[r][t] = in[r][(t + off[r]) mod ncols], t < out_cols. Replaces
+ * kernels.roll_block / roll_block_valid (core/kernels.py:1059-1152) as called by
+ * FilterbankBlock.dedisperse (block.py:267-307) with shifts -delays.
+ * spp_dmt: out[i][t] = sum over c, in channel order, float32, of in[c][(t + off[i][c]) mod
+ * ncols]. Replaces kernels.dmt_block / dmt_block_valid (core/kernels.py:1155-1193) as called by
+ * FilterbankBlock.dmt_transform (block.py:309-343) with shifts +delays.
+ */
+int spp_roll_rows(const float* in, int64_t nrows, int64_t ncols, const int64_t* off_host,
+                  const int64_t* off, int64_t out_cols, float* out, void* stream);
+int spp_dmt(const float* in, int64_t nchans, int64_t ncols, const int64_t* off_host,
+            const int64_t* off, int64_t ndm, int64_t out_cols, float* out, void* stream);
+
+/*
  * Running normalisation of DM series (no reference counterpart: the running mean/std
  * pre-step before MatchedFilter). For each row i of length row_len_host[i]:
  *   z[t] = (x[t] - m[t]) / s[t], m/s = mean/std over x padded symmetrically as

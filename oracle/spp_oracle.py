@@ -377,6 +377,49 @@ def remove_zerodm(stream, nchans: int, bpass, out_dtype, fused=True):
 
 
 # ---------------------------------------------------------------------------
+# in-memory blocks (kernels.py:1059-1193, block.py:267-343)
+# ---------------------------------------------------------------------------
+
+
+def roll_block(arr, shifts):
+    """kernels.roll_block (kernels.py:1059-1099): row r rolled right by shifts[r], circular."""
+    arr = np.asarray(arr)
+    return np.stack([np.roll(arr[r], int(shifts[r])) for r in range(arr.shape[0])])
+
+
+def roll_block_valid(arr, shifts):
+    """kernels.roll_block_valid (kernels.py:1102-1152): the columns no row wraps into."""
+    arr = np.asarray(arr)
+    s = np.asarray(shifts, dtype=np.int64)
+    start, end = max(0, int(s.max())), arr.shape[1] + min(0, int(s.min()))
+    if end - start <= 0:
+        raise ValueError("Not enough samples")
+    return np.stack([arr[r, start - s[r] : end - s[r]] for r in range(arr.shape[0])])
+
+
+def dmt_block(arr, dm_delays, valid=False):
+    """kernels.dmt_block[_valid] (kernels.py:1155-1193): per DM, the channel sum (float32, in
+    channel order from 0) of the block rolled by +delays. The valid form needs every DM's
+    own valid width to equal the one of the whole table (else ValueError)."""
+    arr = np.asarray(arr, dtype=np.float32)
+    d = np.atleast_2d(np.asarray(dm_delays, dtype=np.int64))
+    if valid:
+        width = arr.shape[1] + min(0, int(d.min())) - max(0, int(d.max()))
+        if width <= 0:
+            raise ValueError("Not enough samples")
+    rows = []
+    for drow in d:
+        r = roll_block_valid(arr, drow) if valid else roll_block(arr, drow)
+        if valid and r.shape[1] != width:
+            raise ValueError("could not broadcast")
+        acc = np.zeros(r.shape[1], dtype=np.float32)
+        for c in range(r.shape[0]):
+            acc = acc + r[c]
+        rows.append(acc)
+    return np.stack(rows)
+
+
+# ---------------------------------------------------------------------------
 # matched filter (core/filters.py, core/kernels.py:861-1027, core/stats.py)
 # ---------------------------------------------------------------------------
 

@@ -69,6 +69,30 @@ class FilReader(Filterbank):
         """Bytes per time sample (readers.py:93-96)."""
         return int(self.header.nchans * self.chan_stride)
 
+    def read_block(self, start: int, nsamps: int, fch1: float | None = None,
+                   nchans: int | None = None):
+        """Read ``nsamps`` samples from ``start`` as a channel-major FilterbankBlock of the
+        channels from ``fch1`` (readers.py:98-130)."""
+        from sigpyproc3_amd.block import FilterbankBlock
+
+        hdr = self.header
+        fch1 = fch1 if fch1 is not None else hdr.fch1
+        nchans = nchans if nchans is not None else hdr.nchans
+        if fch1 > hdr.fch1 or nchans > hdr.nchans:
+            msg = f"requested block is out of range: fch1={fch1}, nchans={nchans}"
+            raise ValueError(msg)
+        if start < 0 or start + nsamps > hdr.nsamples:
+            msg = f"requested block is out of range: start={start}, nsamps={nsamps}"
+            raise ValueError(msg)
+        self._file.seek(start * self.samp_stride)
+        data = self._file.cread(hdr.nchans * nsamps)
+        nread = data.size // hdr.nchans
+        data = data.reshape(nread, hdr.nchans).transpose()
+        c0 = int((fch1 - hdr.fch1) / hdr.foff)
+        new_hdr = hdr.new_header({"tstart": hdr.mjd_after_nsamps(start), "nsamples": nread,
+                                  "fch1": fch1, "nchans": nchans})
+        return FilterbankBlock(data[c0 : c0 + nchans], new_hdr)
+
     def read_plan(self, gulp: int = 16384, start: int = 0, nsamps: int | None = None,
                   skipback: int = 0, description: str | None = None, *, quiet: bool = False,
                   allocator=None):

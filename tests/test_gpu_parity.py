@@ -501,3 +501,78 @@ def test_gulp_pipeline_matches_single_buffer_and_stops_early(filfiles):
     next(it)
     it.close()
     assert not pipe.thread.is_alive()
+
+
+# ---------------------------------------------------------------------------
+# in-memory blocks: FilterbankBlock.dedisperse / dmt_transform, read_block
+# ---------------------------------------------------------------------------
+
+
+def test_read_block_vs_stream(filfiles):
+    from sigpyproc3_amd import FilReader
+
+    fil = FilReader(filfiles["4bit"])
+    hdr, s, n = _stream(filfiles["4bit"])
+    blk = fil.read_block(100, 1024)
+    np.testing.assert_array_equal(blk.data, s.reshape(n, hdr["nchans"])[100:1124].T.astype(np.float32))
+    assert blk.header.nsamples == 1024 and blk.nchans == hdr["nchans"]
+    with pytest.raises(ValueError, match="out of range"):
+        fil.read_block(n - 10, 100)
+
+
+@pytest.mark.parametrize("dm", [0.0, 50.0, 200.0])
+def test_block_dedisperse_vs_oracle(filfiles, dm, rng):
+    from sigpyproc3_amd import FilReader
+    from sigpyproc3_amd.block import FilterbankBlock
+
+    fil = FilReader(filfiles["8bit"])
+    hdr = fil.header
+    data = rng.normal(0, 50, (hdr.nchans, hdr.nsamples)).astype(np.float32)
+    block = FilterbankBlock(data, hdr)
+    d = np.atleast_1d(hdr.get_dmdelays(dm)).astype(np.int64)
+    out = block.dedisperse(dm)
+    np.testing.assert_array_equal(out.data, O.roll_block(data, -d))
+    assert out.dm == dm and out.data.shape == data.shape
+    out_v = block.dedisperse(dm, only_valid_samples=True)
+    np.testing.assert_array_equal(out_v.data, O.roll_block_valid(data, -d))
+    assert out_v.header.nsamples == hdr.nsamples - int(d.max())
+
+
+def test_block_dedisperse_direction(filfiles):
+    # tests/test_blocks.py:100-124: an impulse at the middle stays at or before the middle
+    from sigpyproc3_amd import FilReader
+    from sigpyproc3_amd.block import FilterbankBlock
+
+    fil = FilReader(filfiles["8bit"])
+    data = np.zeros((fil.header.nchans, fil.header.nsamples), dtype=np.float32)
+    block = FilterbankBlock(data, fil.header)
+    block.data[:, block.nsamples // 2] = 1.0
+    dd = block.dedisperse(50)
+    np.testing.assert_array_equal(dd.data.mean(axis=1), block.data.mean(axis=1))
+    assert dd.data[:, block.nsamples // 2 + 1 :].sum() == 0
+    assert block.dedisperse(-50).data[:, : block.nsamples // 2].sum() == 0
+    with pytest.raises(ValueError):
+        fil.read_block(100, 1024).dedisperse(10000, only_valid_samples=True)
+
+
+@pytest.mark.parametrize("dm", [0.0, 30.0, 120.0])
+def test_dmt_transform_vs_oracle(filfiles, dm, rng):
+    from sigpyproc3_amd import FilReader
+
+    fil = FilReader(filfiles["8bit"])
+    block = fil.read_block(0, 2048)
+    for dmsteps in (16, 63):
+        dmt = block.dmt_transform(dm, dmsteps=dmsteps)
+        dm_arr = dm + np.linspace(-dm, dm, dmsteps)
+        delays = np.atleast_2d(block.header.get_dmdelays(dm_arr)).astype(np.int64)
+        np.testing.assert_array_equal(dmt.data, O.dmt_block(block.data, delays))
+        np.testing.assert_array_equal(dmt.dms, dm_arr.astype(np.float32))
+        assert dmt.header.nchans == 1
+    # the valid-samples form keeps the block's nsamples in its header, so any DM range with
+    # delays fails the length check (block.py:337-343), as in the reference
+    if dm > 0:
+        with pytest.raises(ValueError):
+            block.dmt_transform(dm, dmsteps=8, only_valid_samples=True)
+    else:
+        v = block.dmt_transform(dm, dmsteps=8, only_valid_samples=True)
+        np.testing.assert_array_equal(v.data, O.dmt_block(block.data, np.zeros((8, block.nchans), int), valid=True))

@@ -370,3 +370,53 @@ def test_remove_zerodm_identities(rng):
     b2 = rng.uniform(50, 150, nchans).astype(np.float32)
     y = O.remove_zerodm(x.reshape(-1), nchans, b2, np.float32).reshape(-1, nchans).astype(np.float64)
     np.testing.assert_allclose((y - b2).sum(axis=1), 0.0, atol=1e-3)
+
+
+# ---------------------------------------------------------------------------
+# in-memory blocks (kernels.py:1059-1193)
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("shifts", [np.arange(5), -np.arange(5), np.array([2, -1, 0, 3, -2]),
+                                    np.array([6, -7, 8, -9, 10])])
+def test_roll_block_rows_and_inverse(shifts):
+    # tests/test_kernels.py:472-491
+    arr = np.arange(25).reshape(5, 5).astype(np.float32)
+    rolled = O.roll_block(arr, shifts)
+    for i, s in enumerate(shifts):
+        np.testing.assert_array_equal(rolled[i], np.roll(arr[i], s))
+    np.testing.assert_array_equal(O.roll_block(rolled, -shifts), arr)
+
+
+@pytest.mark.parametrize("shifts", [np.arange(5), -np.arange(5), np.array([2, -1, 0, 3, -2])])
+def test_roll_block_valid_is_unwrapped_slice(shifts):
+    # tests/test_kernels.py:505-524, 526-535
+    arr = np.arange(50).reshape(5, 10).astype(np.float32)
+    lo, hi = max(0, shifts.max()), arr.shape[1] + min(0, shifts.min())
+    np.testing.assert_array_equal(O.roll_block_valid(arr, shifts), O.roll_block(arr, shifts)[:, lo:hi])
+    assert O.roll_block_valid(arr, np.array([-9, -5, -4, -2, -1])).shape == (5, 1)
+    with pytest.raises(ValueError):
+        O.roll_block_valid(np.zeros((10, 19), np.float32), np.arange(10) * 5)
+
+
+def test_dmt_block_kats_and_order(rng):
+    # tests/test_kernels.py:545-575: an impulse column sums to one in every output sample
+    arr = np.zeros((10, 10), dtype=np.float32)
+    arr[:, 5] = 1.0
+    np.testing.assert_array_equal(O.dmt_block(arr, np.arange(10)[None]), np.ones((1, 10), np.float32))
+    arr = np.zeros((10, 19), dtype=np.float32)
+    arr[:, 9] = 1.0
+    np.testing.assert_array_equal(O.dmt_block(arr, np.arange(10)[None], valid=True),
+                                  np.ones((1, 10), np.float32))
+    with pytest.raises(ValueError):                       # rows of different valid widths
+        O.dmt_block(arr, np.stack([np.arange(10), np.zeros(10, int)]), valid=True)
+    # float32 channel order from 0: brute force with scalar adds
+    x = rng.normal(0, 1e4, (7, 30)).astype(np.float32)
+    d = rng.integers(-40, 40, (3, 7))
+    got = O.dmt_block(x, d)
+    for i in range(3):
+        for t in range(30):
+            acc = np.float32(0.0)
+            for c in range(7):
+                acc = np.float32(acc + x[c, (t - d[i, c]) % 30])
+            assert got[i, t] == acc
