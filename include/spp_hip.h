@@ -161,6 +161,22 @@ int spp_remove_zerodm(const void* in, int nbits, int big, int64_t nchans, int64_
                       const float* chanwts, const float* bpass, void* out, void* stream);
 
 /*
+ * Fold one gulp (kernels.fold, core/kernels.py:404-445, as called by Filterbank.fold,
+ * base.py:1109-1207). The host lists the gulp's valid samples by cell key in time order:
+ * key k covers samples[key_start[k] .. key_start[k+1]) and the cells key_pos[k] + b*nbins,
+ * b < nbands; band b holds channels [band_start[b], band_start[b+1]). Each cell continues its
+ * float32 sum (fold) and count over its samples in time order and the band's channels in
+ * channel order, reading x[s + delays[c]][c] of the packed gulp (nbits 1/2/4/8 with bit order
+ * `big`, or 32 for float32); a negative row wraps by nsamps. Host arrays: none; all pointers
+ * are device pointers. fold/count: float32/int32 [nints * nbands * nbins].
+ */
+int spp_fold(const void* in, int nbits, int big, int64_t nchans, int64_t nsamps,
+             const int32_t* delays, const int32_t* samples, int64_t nlisted,
+             const int32_t* key_start, const int32_t* key_pos, int32_t nkeys,
+             const int32_t* band_start, int32_t nbands, int32_t nbins, float* fold, int32_t* count,
+             void* stream);
+
+/*
  * In-memory blocks: float32 [nrows][ncols] (channel-major FilterbankBlock data). Row r is
  * read at column (t + off[r]) mod ncols; off_host and off hold the same int64 offsets (host
  * copy for validation, device copy for the kernel), each in [0, ncols). out_cols < ncols is

@@ -377,6 +377,59 @@ def remove_zerodm(stream, nchans: int, bpass, out_dtype, fused=True):
 
 
 # ---------------------------------------------------------------------------
+# fold (kernels.py:404-445, base.py:1109-1207)
+# ---------------------------------------------------------------------------
+
+
+def fold_kernel(inarray, fold_ar, count_ar, delays, maxdelay, tsamp, period, accel, total_nsamps,
+                nsamps, nchans, nbins, nints, nsubs, index):
+    """kernels.fold (kernels.py:404-445) in plain Python floats (IEEE double, like numba
+    without fastmath). tsamp, period and accel arrive as float32 (the numba signature)."""
+    tsamp, period, accel = float(np.float32(tsamp)), float(np.float32(period)), float(np.float32(accel))
+    factor1 = total_nsamps / nints
+    factor2 = nchans / nsubs
+    tobs = total_nsamps * tsamp
+    x = np.asarray(inarray)
+    chans = np.arange(nchans)
+    sub_band = np.array([int(c // factor2) for c in range(nchans)])
+    for isamp in range(nsamps - maxdelay):
+        tj = (isamp + index) * tsamp
+        phase = nbins * tj * (1 + accel * (tj - tobs) / (2 * 299792458.0)) / period + 0.5
+        phasebin = abs(int(phase)) % nbins
+        subint = (isamp + index) // factor1
+        pos1 = (subint * nbins * nsubs) + phasebin
+        pos2 = (pos1 + sub_band * nbins).astype(np.int64)
+        rows = isamp + np.asarray(delays, dtype=np.int64)
+        idx = nchans * rows + chans
+        idx = np.where(idx < 0, idx + x.size, idx)
+        np.add.at(fold_ar, pos2, x[idx].astype(np.float32))
+        np.add.at(count_ar, pos2, 1)
+
+
+def filterbank_fold(stream, nsamples, nchans, tsamp, delays, period, accel=0.0, nbins=50, nints=32,
+                    nbands=32, gulp=16384, start=0, nsamps=None):
+    """Filterbank.fold (base.py:1109-1207) over the unpacked stream; returns the
+    [nints, nbands, nbins] cube (float32)."""
+    nbands = min(nbands, nchans)
+    delays = np.asarray(delays, dtype=np.int32)
+    max_delay = int(delays.max())
+    gulp = max(2 * max_delay, gulp)
+    fold_ar = np.zeros(nbins * nints * nbands, dtype=np.float32)
+    count_ar = np.zeros(nbins * nints * nbands, dtype=np.int32)
+    n = (nsamples - start) if nsamps is None else nsamps
+    x = np.asarray(stream).reshape(-1, nchans)
+    pos = start
+    for ii, nsamps_r, skip in read_plan_blocks(n, gulp, max_delay):
+        data = x[pos : pos + nsamps_r].reshape(-1)
+        fold_kernel(data, fold_ar, count_ar, delays, max_delay, tsamp, period, accel, nsamples,
+                    nsamps_r, nchans, nbins, nints, nbands, ii * (gulp - max_delay))
+        pos += nsamps_r + skip
+    with np.errstate(divide="ignore", invalid="ignore"):
+        fold_ar /= count_ar
+    return fold_ar.reshape(nints, nbands, nbins)
+
+
+# ---------------------------------------------------------------------------
 # in-memory blocks (kernels.py:1059-1193, block.py:267-343)
 # ---------------------------------------------------------------------------
 

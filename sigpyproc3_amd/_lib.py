@@ -60,6 +60,7 @@ SIGNATURES: dict[str, tuple[object, list]] = {
     "spp_bandpass": (_int, [_p, _int, _int, _i64, _i64, _p, _int, _p]),
     "spp_mask_bytes": (_int, [_p, _i64, _i64, _p, _p, _p]),
     "spp_remove_zerodm": (_int, [_p, _int, _int, _i64, _i64, _p, _p, _p, _p]),
+    "spp_fold": (_int, [_p, _int, _int, _i64, _i64, _p, _p, _i64, _p, _p, _i32, _p, _i32, _i32, _p, _p, _p]),
     "spp_roll_rows": (_int, [_p, _i64, _i64, _p, _p, _i64, _p, _p]),
     "spp_dmt": (_int, [_p, _i64, _i64, _p, _p, _i64, _i64, _p, _p]),
     "spp_running_normalise_workspace_size": (_int, [_i64, _p, _i32, _P64]),

@@ -386,6 +386,65 @@ class Filterbank(ABC):
             out_file.close()
         return outfile_name
 
+    def fold(self, period: float, dm: float, accel: float = 0, nbins: int = 50, nints: int = 32,
+             nbands: int = 32, gulp: int = 16384, start: int = 0, nsamps: int | None = None,
+             **plan_kwargs):
+        """Fold into a (subint, subband, phase) cube (base.py:1109-1207) on the GPU.
+
+        Per gulp the host lists the valid samples by (subint, phase bin) with the
+        reference's float64 arithmetic (``foldedcube.fold_cells``); ``spp_fold`` then sums
+        every cell in the reference's order. Returns ``FoldedData``.
+        """
+        from sigpyproc3_amd.foldedcube import FoldedData, band_starts, fold_cells
+
+        hdr = self.header
+        if np.isclose(np.modf(period / hdr.tsamp)[0], 0.001, atol=0):
+            self.logger.warning("Foldng interval is an integer multiple of the sampling time")
+        if nbins > period / hdr.tsamp:
+            self.logger.warning("Number of phase bins is greater than period/sampling time")
+        if (hdr.nsamples * hdr.nchans) // (nbands * nints * nbins) < 10:
+            msg = f"nbands x nints x nbins is too large: {nbands * nints * nbins}"
+            raise ValueError(msg)
+        nbands = min(nbands, hdr.nchans)
+        chan_delays = np.atleast_1d(hdr.get_dmdelays(dm)).astype(np.int32)
+        max_delay = int(chan_delays.max())
+        gulp = max(2 * max_delay, gulp)
+        nbits, big = self._kernel_nbits()
+        _lib.require_gpu()
+        ncell = nbins * nints * nbands
+        fold_d = _lib.zeros((ncell,), "float32")
+        count_d = _lib.zeros((ncell,), "int32")
+        d_dev = _lib.to_device(chan_delays)
+        bands_dev = _lib.to_device(band_starts(hdr.nchans, nbands))
+        stream = _lib.stream_handle()
+        keep = []
+        for nsamps_r, ii, dev, _nb in self._device_gulps(gulp, start, nsamps, plan_kwargs, max_delay):
+            nvalid = nsamps_r - max_delay
+            if nvalid <= 0:
+                continue
+            if int(np.abs(chan_delays).max()) >= nsamps_r:
+                msg = "channel delays exceed the gulp"
+                raise ValueError(msg)
+            subint, phasebin = fold_cells(ii * (gulp - max_delay), nvalid, hdr.nsamples, hdr.tsamp,
+                                          period, accel, nbins, nints)
+            key = subint * nbins + phasebin
+            order = np.argsort(key, kind="stable").astype(np.int32)
+            keys, first = np.unique(key[order], return_index=True)
+            key_start = np.append(first, nvalid).astype(np.int32)
+            key_pos = ((keys // nbins) * nbins * nbands + keys % nbins).astype(np.int32)
+            bufs = [_lib.to_device(order), _lib.to_device(key_start), _lib.to_device(key_pos)]
+            keep.append(bufs)
+            _lib.call("spp_fold", _lib.ptr(dev), nbits, big, hdr.nchans, nsamps_r, _lib.ptr(d_dev),
+                      _lib.ptr(bufs[0]), nvalid, _lib.ptr(bufs[1]), _lib.ptr(bufs[2]), len(keys),
+                      _lib.ptr(bands_dev), nbands, nbins, _lib.ptr(fold_d), _lib.ptr(count_d), stream)
+        fold_ar = fold_d.cpu().numpy()
+        count_ar = count_d.cpu().numpy()
+        del keep
+        with np.errstate(divide="ignore", invalid="ignore"):
+            fold_ar /= count_ar
+        fold_ar = fold_ar.reshape(nints, nbands, nbins)
+        return FoldedData(fold_ar, hdr.new_header(), period, dm, accel)
+
     def dedisperse_sweep(self, dms, ref_freq: str | float = "fch1", start: int = 0,
                          nsamps: int | None = None):
         """Dedisperse at every DM of ``dms`` in one device pass.

@@ -576,3 +576,28 @@ def test_dmt_transform_vs_oracle(filfiles, dm, rng):
     else:
         v = block.dmt_transform(dm, dmsteps=8, only_valid_samples=True)
         np.testing.assert_array_equal(v.data, O.dmt_block(block.data, np.zeros((8, block.nchans), int), valid=True))
+
+
+# ---------------------------------------------------------------------------
+# fold
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("key", ["1bit", "2bit", "4bit", "8bit", "tutorial"])
+def test_fold_vs_oracle(filfiles, key):
+    from sigpyproc3_amd import FilReader
+
+    path = _fixture(filfiles, key)
+    hdr, s, n = _stream(path)
+    fil = FilReader(path)
+    for period, dm, accel, gulp in ((0.0123, 20.0, 0.0, 1500), (0.00714, 0.0, 3.5, 16384)):
+        folded = fil.fold(period, dm, accel=accel, nbins=16, nints=8, nbands=8, gulp=gulp)
+        d = np.atleast_1d(fil.header.get_dmdelays(dm))
+        ref = O.filterbank_fold(s, n, hdr["nchans"], hdr["tsamp"], d, period, accel=accel, nbins=16,
+                                nints=8, nbands=8, gulp=gulp)
+        np.testing.assert_array_equal(folded.data, ref)
+        assert (folded.nsubints, folded.nsubbands, folded.nbins) == (8, 8, 16)
+        prof = folded.get_profile().data
+        np.testing.assert_array_equal(prof, np.sum(ref, axis=(0, 1)))
+    with pytest.raises(ValueError, match="too large"):
+        fil.fold(0.01, 0.0, nbins=4096, nints=4096, nbands=64)

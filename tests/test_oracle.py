@@ -420,3 +420,34 @@ def test_dmt_block_kats_and_order(rng):
             for c in range(7):
                 acc = np.float32(acc + x[c, (t - d[i, c]) % 30])
             assert got[i, t] == acc
+
+
+# ---------------------------------------------------------------------------
+# fold (kernels.py:404-445)
+# ---------------------------------------------------------------------------
+
+
+def test_fold_conserves_sums_and_finds_the_pulse(rng):
+    nchans, n, tsamp, period = 8, 6000, 1e-3, 0.025
+    x = rng.integers(0, 10, (n, nchans)).astype(np.uint8)
+    x[::25, :] += 100                                   # one pulse per period (25 samples)
+    delays = np.zeros(nchans, dtype=np.int32)
+    fold_ar = np.zeros(20 * 4 * 2, dtype=np.float32)
+    count_ar = np.zeros(20 * 4 * 2, dtype=np.int32)
+    O.fold_kernel(x.reshape(-1), fold_ar, count_ar, delays, 0, tsamp, period, 0.0, n, n, nchans, 20, 4, 2, 0)
+    assert count_ar.sum() == n * nchans                  # every sample of every channel once
+    assert fold_ar.sum() == float(x.astype(np.int64).sum())   # integer sums < 2^24: exact
+    cube = (fold_ar / count_ar).reshape(4, 2, 20)
+    prof = cube.sum(axis=(0, 1))
+    # pulses at t = 25k land in phase bin int(20 * k + 0.5) % 20 = 0
+    assert int(np.argmax(prof)) == 0 and prof[0] > 5 * np.median(prof)
+
+
+def test_filterbank_fold_matches_gulp_free_fold(rng):
+    """Gulps with skipback give the same cube as one pass (the schedule tiles time)."""
+    nchans, n = 16, 9000
+    x = rng.integers(0, 4, (n, nchans)).astype(np.uint8)
+    d = np.sort(rng.integers(0, 300, nchans))[::-1].astype(np.int32)
+    a = O.filterbank_fold(x.reshape(-1), n, nchans, 64e-6, d, 0.0123, nbins=16, nints=8, nbands=4, gulp=1000)
+    b = O.filterbank_fold(x.reshape(-1), n, nchans, 64e-6, d, 0.0123, nbins=16, nints=8, nbands=4, gulp=n)
+    np.testing.assert_array_equal(a, b)
