@@ -38,8 +38,16 @@ constexpr int CT_TAPS = 2048;    // largest template length
 
 int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
+// i mod m for 0 <= i < k*m with small k, by subtraction (64-bit division is a long
+// instruction sequence on the GPU).
+__device__ __forceinline__ int64_t wrap_index(int64_t i, int64_t m) {
+    while (i >= m) i -= m;
+    return i;
+}
+
 __device__ __forceinline__ int64_t sym_index(int64_t i, int64_t n) {
     // numpy 'symmetric' extension: period 2n, mirrored including the edge sample.
+    if (i >= 0 && i < n) return i;  // interior: no reflection (and no 64-bit division)
     const int64_t two_n = 2 * n;
     int64_t m = i % two_n;
     if (m < 0) m += two_n;
@@ -246,7 +254,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_circ_sum(const float* __restrict
     const MfRow r = rows[row];
     const float* __restrict__ zr = z + row * ldz;
     double s = 0.0;
-    for (int64_t i = threadIdx.x; i < r.ng; i += MF_THREADS) s += static_cast<double>(zr[i % r.n]);
+    for (int64_t i = threadIdx.x; i < r.ng; i += MF_THREADS) s += static_cast<double>(zr[wrap_index(i, r.n)]);
     s = block_sum(s, red);
     if (threadIdx.x == 0) S[row] = s;
 }
@@ -289,8 +297,8 @@ __global__ __launch_bounds__(MF_THREADS) void k_boxcar_part(
         const int k = tid * KE + e;
         double v = 0.0;
         if (k < nel) {
-            int64_t i = (a + k) % r.ng;
-            v = static_cast<double>(zr[i % r.n]);
+            const int64_t i = wrap_index(a + k, r.ng);   // a + k < n + MF_B + MF_WMAX
+            v = static_cast<double>(zr[wrap_index(i, r.n)]);  // ng < 2n
         }
         tot += v;
         loc[e] = tot;
@@ -365,9 +373,10 @@ __global__ __launch_bounds__(MF_THREADS) void k_template_conv(
     // zt[(t - ref + m) mod Ng] for t in [a, a + CT_B), m < len.
     const int nel = CT_B + td.len - 1;
     for (int k = tid; k < nel; k += MF_THREADS) {
-        int64_t i = (a - td.ref + k) % ng;
+        int64_t i = a - td.ref + k;  // in [-ref, n + CT_B + CT_TAPS)
         if (i < 0) i += ng;
-        zs[k] = z[i % n];
+        i = wrap_index(i, ng);
+        zs[k] = z[wrap_index(i, n)];  // ng < 2n
     }
     for (int m = tid; m < td.len; m += MF_THREADS) hs[m] = taps[td.off + m];
     __syncthreads();
@@ -385,7 +394,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_circ_sum1(const float* __restric
                                                           int64_t ng, double* __restrict__ S) {
     __shared__ double red[MF_THREADS];
     double s = 0.0;
-    for (int64_t i = threadIdx.x; i < ng; i += MF_THREADS) s += static_cast<double>(z[i % n]);
+    for (int64_t i = threadIdx.x; i < ng; i += MF_THREADS) s += static_cast<double>(z[wrap_index(i, n)]);
     s = block_sum(s, red);
     if (threadIdx.x == 0) S[0] = s;
 }
