@@ -40,7 +40,7 @@ constexpr int SW_S = 256;                // interleave stride = words per wave r
 constexpr int SW_DT = 16;                // DM rows per tile
 constexpr int SW_LW = 496;               // LDS words per staged channel
 constexpr int SW_SPREAD_MAX = SW_LW - SW_S - 4;
-constexpr int G_TJ = 32;                 // words per build block (time)
+constexpr int B_TJ = 128;                // words per build block (time)
 constexpr int G_STRIDE = 68;             // bytes per staged row in the build kernel
 
 template <int NBITS>
@@ -48,51 +48,73 @@ struct Fields {
     static constexpr int IL = NBITS <= 2 ? 8 : 4;          // samples per word
     static constexpr int FB = 32 / IL;                     // bits per field
     static constexpr uint32_t MAXV = NBITS == 8 ? 255u : ((1u << NBITS) - 1u);
-    static constexpr int CB = 512 / NBITS;                 // build: channels per 64-byte row chunk
 };
 
-// Stage 1. Block (bx, by): words [bx*32, bx*32+32) of channels [by*CB, by*CB+CB); only the
+// Stage 1. Block (bx, by): words [bx*B_TJ, bx*B_TJ + B_TJ) of the channels in row bytes
+// [by*64, by*64 + 64) (64 * 8/NBITS channels). The rows j0 + k*S + i (k < IL, i < B_TJ) are
+// staged in LDS; thread (quad q, byte column bc) then builds words j0 + 4q .. 4q+3 of every
+// channel packed in byte column bc from one LDS byte per (row, word) and stores them as
+// one 16-byte write per channel (a wave writes 512 contiguous bytes per channel). Only the
 // first glen words of each channel row are written (the sweep reads no further).
-template <int NBITS, bool BIG>
+template <int NBITS, bool BIG, bool ALIGNED>
 __global__ __launch_bounds__(256) void k_build_g(const uint8_t* __restrict__ in, int64_t nsamps,
                                                  int64_t nchans, uint32_t* __restrict__ G,
                                                  int64_t gpitch, int64_t glen) {
     using F = Fields<NBITS>;
-    constexpr int ROWS = F::IL * G_TJ;
+    constexpr int IL = F::IL;
+    constexpr int PER = (NBITS < 8) ? 8 / NBITS : 1;     // channels per byte
+    constexpr int ROWS = IL * B_TJ;
     __shared__ uint8_t raw[ROWS * G_STRIDE];
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int64_t j0 = static_cast<int64_t>(blockIdx.x) * G_TJ;
-    const int64_t c0 = static_cast<int64_t>(blockIdx.y) * F::CB;
+    const int tid = threadIdx.x;
+    const int64_t j0 = static_cast<int64_t>(blockIdx.x) * B_TJ;
     const int64_t row_bytes = nchans * NBITS / 8;
-    const int64_t b0 = c0 * NBITS / 8;
-    // Load the 64-byte row chunks of rows j0 + k*S + i (k < IL, i < 32): lane = byte.
-    for (int r = wave; r < ROWS; r += 4) {
-        const int64_t row = j0 + static_cast<int64_t>(r / G_TJ) * SW_S + (r % G_TJ);
-        uint8_t v = 0;
-        if (row < nsamps && b0 + lane < row_bytes) v = in[row * row_bytes + b0 + lane];
-        raw[r * G_STRIDE + lane] = v;
+    const int64_t b0 = static_cast<int64_t>(blockIdx.y) * 64;
+    // Stage: element e = (row r, dword q of the row's 64 bytes); 4 byte loads per thread.
+    for (int e = tid; e < ROWS * 16; e += 256) {
+        const int r = e >> 4, q = e & 15;
+        const int64_t row = j0 + static_cast<int64_t>(r / B_TJ) * SW_S + (r % B_TJ);
+        uint32_t v = 0;
+        if (row < nsamps) {
+            const uint8_t* src = in + row * row_bytes + b0 + 4 * q;
+            const int64_t left = row_bytes - (b0 + 4 * q);
+            if (ALIGNED) {
+                // row_bytes and the stream start are multiples of 4: whole dwords or none
+                if (left > 0) v = *reinterpret_cast<const uint32_t*>(src);
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (u < left) v |= static_cast<uint32_t>(src[u]) << (8 * u);
+            }
+        }
+        *reinterpret_cast<uint32_t*>(&raw[r * G_STRIDE + 4 * q]) = v;
     }
     __syncthreads();
-    // Assemble G[c0 + cl][j0 + i]: lanes = 32 consecutive words of 2 channels.
-    const int i = threadIdx.x & (G_TJ - 1);
-    const int64_t j = j0 + i;
-    for (int cl = threadIdx.x / G_TJ; cl < F::CB; cl += 256 / G_TJ) {
-        const int64_t c = c0 + cl;
-        if (c >= nchans) break;
-        const int byte = (cl * NBITS) / 8;
-        int shift = 0;
-        if constexpr (NBITS < 8) {
-            const int k = cl % (8 / NBITS);
-            shift = BIG ? (8 - (k + 1) * NBITS) : k * NBITS;
-        }
-        uint32_t w = 0;
+    const int qd = tid & 31;
+    const int64_t j = j0 + 4 * qd;
+    if (j >= glen) return;
+    for (int bc = tid >> 5; bc < 64; bc += 8) {
+        uint32_t w[PER][4];
 #pragma unroll
-        for (int k = 0; k < F::IL; ++k) {
-            const uint32_t v = (static_cast<uint32_t>(raw[(k * G_TJ + i) * G_STRIDE + byte]) >> shift) & F::MAXV;
-            w |= v << (F::FB * k);
+        for (int p = 0; p < PER; ++p)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) w[p][u] = 0;
+#pragma unroll
+        for (int k = 0; k < IL; ++k)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t byte = raw[(k * B_TJ + 4 * qd + u) * G_STRIDE + bc];
+#pragma unroll
+                for (int p = 0; p < PER; ++p) {
+                    const int shift = (NBITS == 8) ? 0 : (BIG ? (8 - (p + 1) * NBITS) : p * NBITS);
+                    w[p][u] |= ((byte >> shift) & F::MAXV) << (F::FB * k);
+                }
+            }
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int64_t c = (b0 + bc) * PER + p;
+            if (c < nchans)
+                *reinterpret_cast<uint4*>(G + c * gpitch + j) = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
         }
-        if (j < glen) G[c * gpitch + j] = w;
     }
 }
 
@@ -606,10 +628,15 @@ Extent extent_of(const spp_sweep_plan* p, int64_t n) {
 template <int NBITS, bool BIG>
 int launch_build(const uint8_t* packed, const spp_sweep_plan* p, const Extent& e, uint32_t* G,
                  hipStream_t st) {
-    dim3 grid(static_cast<unsigned>(spp::ceil_div(e.glen, G_TJ)),
-              static_cast<unsigned>(spp::ceil_div(p->nchans, Fields<NBITS>::CB)));
-    hipLaunchKernelGGL((k_build_g<NBITS, BIG>), grid, dim3(256), 0, st, packed, e.n, p->nchans, G,
-                       p->gpitch, e.glen);
+    dim3 grid(static_cast<unsigned>(spp::ceil_div(e.glen, int64_t(B_TJ))),
+              static_cast<unsigned>(spp::ceil_div(p->nchans * NBITS / 8, int64_t(64))));
+    const int64_t row_bytes = p->nchans * NBITS / 8;
+    if (row_bytes % 4 == 0 && reinterpret_cast<uintptr_t>(packed) % 4 == 0)
+        hipLaunchKernelGGL((k_build_g<NBITS, BIG, true>), grid, dim3(256), 0, st, packed, e.n, p->nchans, G,
+                           p->gpitch, e.glen);
+    else
+        hipLaunchKernelGGL((k_build_g<NBITS, BIG, false>), grid, dim3(256), 0, st, packed, e.n, p->nchans, G,
+                           p->gpitch, e.glen);
     SPP_LAUNCHED("k_build_g");
     return SPP_OK;
 }
