@@ -245,6 +245,87 @@ def bench_mf(args) -> int:
     return 0
 
 
+def bench_single(args) -> int:
+    """The one-DM path at 8-bit, 1024 ch, 65 536 samples, DM 300 (the gulp schedule of
+    Filterbank.dedisperse: gulp 16384, skipback = max delay), with the gulps already in HBM,
+    and the 2-bit unpack of a 256 MiB packed buffer. Both are HBM-bound: the roofline is
+    (bytes read + bytes written) per launch / kernel time against 8 TB/s."""
+    import torch
+
+    from sigpyproc3_amd import _lib
+    from sigpyproc3_amd.readers import plan_blocks
+    from sigpyproc3_amd.synth import cfg_single_dm, generate_packed
+
+    _lib.load()
+    _lib.require_gpu()
+    stream = _lib.stream_handle()
+    cfg = cfg_single_dm()
+    delays = cfg.delays([300.0])[0].astype(np.int32)
+    md = int(delays.max())
+    packed = generate_packed(cfg, device=_lib.device())
+    d_dev = _lib.to_device(delays)
+    gulp = max(2 * md, 16384)
+    _, blocks = plan_blocks(cfg.nsamps, gulp, md)
+    tim = _lib.zeros((cfg.nsamps - md,), "float32")
+    starts, pos = [], 0
+    for ii, n, skip in blocks:
+        starts.append((ii, n, pos))
+        pos += n + skip
+
+    def one_pass():
+        for ii, n, p0 in starts:
+            _lib.call("spp_dedisperse", _lib.ptr(packed[p0 * cfg.nchans:]), 8, 0, cfg.nchans, n,
+                      _lib.ptr(d_dev), md, _lib.ptr(tim), ii * (gulp - md), stream)
+
+    for _ in range(max(1, args.warmup)):
+        one_pass()
+    torch.cuda.synchronize()
+    reps = 50
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        one_pass()
+    e1.record()
+    torch.cuda.synchronize()
+    t_pass = e0.elapsed_time(e1) / reps * 1e-3
+    bytes_pass = sum(n * cfg.nchans for _, n, _ in starts) + 4 * (cfg.nsamps - md)
+    row = tim.cpu().numpy()
+    peak_t = int(np.argmax(row))
+    # unpack: 2-bit, 256 MiB packed -> 1 GiB
+    nb = 256 << 20
+    src = torch.randint(0, 256, (nb,), dtype=torch.uint8, device=_lib.device())
+    dst = _lib.empty((nb * 4,), "uint8")
+    _lib.call("spp_unpack", _lib.ptr(src), nb, _lib.ptr(dst), 2, 1, stream)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(10):
+        _lib.call("spp_unpack", _lib.ptr(src), nb, _lib.ptr(dst), 2, 1, stream)
+    e1.record()
+    torch.cuda.synchronize()
+    t_unpack = e0.elapsed_time(e1) / 10 * 1e-3
+    result = {
+        "metric": "one-DM dedispersion (gulp path) and 2-bit unpack: HBM-bound kernels",
+        "value": (cfg.nsamps - md) / t_pass,
+        "unit": "output samples/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "8-bit, 1024 ch x 65536 samples, DM 300 (max delay %d), gulp %d, %d gulps"
+                               % (md, gulp, len(starts))},
+        "dedisperse": {"us_per_pass": t_pass * 1e6, "hbm_bytes_per_pass": bytes_pass,
+                       "hbm_gbs": bytes_pass / t_pass / 1e9, "hbm_frac": bytes_pass / t_pass / 8e12,
+                       "peak_sample": peak_t, "pulse_at": [20000, 20015]},
+        "unpack_2bit": {"ms": t_unpack * 1e3, "bytes": nb * 5, "hbm_gbs": nb * 5 / t_unpack / 1e9,
+                        "hbm_frac": nb * 5 / t_unpack / 8e12},
+    }
+    log(f"single: {t_pass * 1e6:.1f} us per dedisperse pass ({result['dedisperse']['hbm_gbs']:.0f} GB/s), "
+        f"unpack {result['unpack_2bit']['hbm_gbs']:.0f} GB/s")
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def bench_file(args) -> int:
     """The 2-bit 4096-ch 2^20-sample stream as a SIGPROC file (1 GiB of data): one-DM
     Filterbank.dedisperse through the double-buffered gulp pipeline and through a single
@@ -321,11 +402,11 @@ def main() -> int:
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="first run a plain device copy of this many GiB three times (the "
                          "reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--workload", choices=("sweep", "serve", "mf", "file"), default="sweep",
+    ap.add_argument("--workload", choices=("sweep", "serve", "mf", "file", "single"), default="sweep",
                     help="sweep: the 2-bit 2048-DM sweep (the headline line); serve: the "
                          "16-stream gulp service with p50/p99 latency; mf: the 8-bit 1024-DM "
                          "sweep + normalisation + boxcar matched filter; file: the same stream "
-                         "read from a SIGPROC file")
+                         "read from a SIGPROC file; single: the one-DM gulp path and unpack")
     ap.add_argument("--serve-gulps", type=int, default=160, help="gulps per serve run")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -339,6 +420,8 @@ def main() -> int:
         return bench_mf(args)
     if args.workload == "file":
         return bench_file(args)
+    if args.workload == "single":
+        return bench_single(args)
 
     import torch
 

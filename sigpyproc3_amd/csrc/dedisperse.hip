@@ -24,6 +24,7 @@ constexpr int DD_T = 512;       // output samples per block (2 per thread)
 constexpr int DD_CC = 64;       // channels per stage at most
 constexpr int DD_ROWS = 768;    // staged rows per stage: spread <= DD_ROWS - DD_T
 constexpr int DD_STRIDE = 68;   // bytes per staged row
+constexpr int DD_LD = 16;       // staged rows in flight per wave
 constexpr int DDF_CC = 16;      // float32 path: channels per stage
 constexpr int DDF_ROWS = 768;
 constexpr int DDF_STRIDE = 17;  // floats per staged row
@@ -103,12 +104,24 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
         const int nrows = DD_T + (dmax - dmin);
         const int64_t rbase = t0 + dmin;
         if (lane < n) {
-            for (int r = wave; r < nrows; r += DD_THREADS / 64) {
-                int64_t row = rbase + r;
-                if (row < 0) row += nsamps;  // numba wraparound of a negative index
-                uint32_t v = 0;
-                if (row >= 0 && row < nsamps) v = packed_sample<NBITS, BIG>(in, row, row_bytes, c + lane);
-                tile[r * DD_STRIDE + lane] = static_cast<uint8_t>(v);
+            // DD_LD rows per wave in flight: the loads are independent, the stores follow.
+            constexpr int WV = DD_THREADS / 64;
+            for (int r0 = wave; r0 < nrows; r0 += DD_LD * WV) {
+                uint32_t v[DD_LD];
+#pragma unroll
+                for (int u = 0; u < DD_LD; ++u) {
+                    const int r = r0 + u * WV;
+                    int64_t row = rbase + r;
+                    if (row < 0) row += nsamps;  // numba wraparound of a negative index
+                    v[u] = 0;
+                    if (r < nrows && row >= 0 && row < nsamps)
+                        v[u] = packed_sample<NBITS, BIG>(in, row, row_bytes, c + lane);
+                }
+#pragma unroll
+                for (int u = 0; u < DD_LD; ++u) {
+                    const int r = r0 + u * WV;
+                    if (r < nrows) tile[r * DD_STRIDE + lane] = static_cast<uint8_t>(v[u]);
+                }
             }
         }
         __syncthreads();
@@ -242,8 +255,8 @@ int launch_int(const uint8_t* in, int nbits, int big, const int32_t* delays, con
     int64_t groups = 1;
     const int64_t maxval = (nbits == 8) ? 255 : ((1 << nbits) - 1);
     if (nchans * maxval < (int64_t(1) << 24)) {
-        groups = std::max<int64_t>(1, std::min<int64_t>(spp::ceil_div(4 * cus, bx),
-                                                        spp::ceil_div(nchans, DD_CC)));
+        groups = std::max<int64_t>(1, std::min<int64_t>(spp::ceil_div(8 * cus, bx),
+                                                        spp::ceil_div(nchans, int64_t(16))));
     }
     int64_t cpg = spp::ceil_div(nchans, groups);
     cpg = spp::ceil_div(cpg, 8) * 8;

@@ -29,7 +29,7 @@ namespace {
 constexpr int RN_SEG = 2048;     // padded samples per segment / outputs per k_norm block
 constexpr int RN_THREADS = 256;
 constexpr int RN_PER = RN_SEG / RN_THREADS;  // 8
-constexpr int MF_B = 2048;       // outputs per MF block
+constexpr int MF_B = 4096;       // outputs per MF block
 constexpr int MF_THREADS = 256;
 constexpr int MF_WMAX = 1024;    // largest boxcar width supported
 constexpr int MF_MAXW = 32;      // largest number of widths
@@ -54,43 +54,53 @@ __device__ __forceinline__ int64_t sym_index(int64_t i, int64_t n) {
     return (m < n) ? m : (two_n - 1 - m);
 }
 
-// Block-wide sum of a double (fixed tree order). All threads get the result.
+// Block-wide sum of a double in a fixed order (butterfly within each wave, then the wave
+// totals in wave order). All threads get the result. blockDim.x is a multiple of 64.
 __device__ double block_sum(double v, double* red) {
-    const int tid = threadIdx.x;
-    red[tid] = v;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[wave] = v;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (tid < s) red[tid] += red[tid + s];
-        __syncthreads();
-    }
-    const double r = red[0];
+    double r = 0.0;
+    for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) r += red[w];
     __syncthreads();
     return r;
 }
 
-// Exclusive block scan of two doubles per thread (Hillis-Steele on LDS, fixed order).
-__device__ void block_exscan2(double& a, double& b, double* sa, double* sb) {
-    const int tid = threadIdx.x;
-    sa[tid] = a;
-    sb[tid] = b;
-    __syncthreads();
-    for (int off = 1; off < static_cast<int>(blockDim.x); off <<= 1) {
-        double ta = 0.0, tb = 0.0;
-        if (tid >= off) {
-            ta = sa[tid - off];
-            tb = sb[tid - off];
-        }
-        __syncthreads();
-        if (tid >= off) {
-            sa[tid] += ta;
-            sb[tid] += tb;
-        }
-        __syncthreads();
+// Inclusive scan of a double across one wave (fixed order).
+__device__ __forceinline__ double wave_incl_scan(double v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
     }
-    const double ia = sa[tid] - a, ib = sb[tid] - b;  // inclusive - own = exclusive
+    return v;
+}
+
+// Exclusive block scan of two doubles per thread: within each wave by shuffles, then the
+// preceding waves' totals from LDS. Two barriers; sa/sb hold one entry per wave.
+__device__ void block_exscan2(double& a, double& b, double* sa, double* sb) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double ia = wave_incl_scan(a), ib = wave_incl_scan(b);
+    double xa = __shfl_up(ia, 1, 64), xb = __shfl_up(ib, 1, 64);
+    if (lane == 0) {
+        xa = 0.0;
+        xb = 0.0;
+    }
+    if (lane == 63) {
+        sa[wave] = ia;
+        sb[wave] = ib;
+    }
     __syncthreads();
-    a = ia;
-    b = ib;
+    double pa = 0.0, pb = 0.0;
+    for (int w = 0; w < wave; ++w) {
+        pa += sa[w];
+        pb += sb[w];
+    }
+    __syncthreads();
+    a = pa + xa;
+    b = pb + xb;
 }
 
 // ---------------------------------------------------------------- running normalise

@@ -43,6 +43,8 @@ constexpr int SW_SPREAD_MAX = SW_LW - SW_S - 4;
 constexpr int B_TJ = 128;                // words per build block (time)
 constexpr int G_STRIDE = 68;             // bytes per staged row in the build kernel
 
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+
 template <int NBITS>
 struct Fields {
     static constexpr int IL = NBITS <= 2 ? 8 : 4;          // samples per word
@@ -112,8 +114,11 @@ __global__ __launch_bounds__(256) void k_build_g(const uint8_t* __restrict__ in,
 #pragma unroll
         for (int p = 0; p < PER; ++p) {
             const int64_t c = (b0 + bc) * PER + p;
-            if (c < nchans)
-                *reinterpret_cast<uint4*>(G + c * gpitch + j) = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
+            if (c < nchans) {
+                // Streaming store: G (GiBs) is re-read by the sweep much later, not from cache.
+                const v4u32 v = {w[p][0], w[p][1], w[p][2], w[p][3]};
+                __builtin_nontemporal_store(v, reinterpret_cast<v4u32*>(G + c * gpitch + j));
+            }
         }
     }
 }
