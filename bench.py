@@ -341,7 +341,7 @@ def bench_file(args) -> int:
     _lib.require_gpu()
     cfg = cfg_sweep_2bit()
     with tempfile.TemporaryDirectory() as tmp:
-        path = f"{tmp}/cfg2.fil"
+        path = f"{tmp}/sweep_2bit.fil"
         write_sigproc(path, cfg, generate_packed(cfg, device=_lib.device()))
         torch.cuda.synchronize()
         fil = FilReader(path)
@@ -411,7 +411,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
-              "(the sweep shards by DM list across GPUs; see DESIGN.md)", file=sys.stderr)
+              "(the sweep would shard by DM list, one process per GPU)", file=sys.stderr)
         return 1
 
     if args.workload == "serve":
@@ -501,7 +501,7 @@ def main() -> int:
 
     adds = float(total) * cfg.nchans
     achieved_t = adds / (sweep_ms.mean() * 1e-3) / 1e12
-    traffic = read_traffic(ROOT / "profiles" / "k_sweep_cfg2_traffic.txt")
+    traffic = read_traffic(ROOT / "profiles" / "k_sweep_2bit_4096ch_2048dm_traffic.txt")
     roofline = {
         "bound": "valu",
         "achieved": round(achieved_t, 3),

@@ -128,7 +128,7 @@ def write_sigproc(path, cfg: SynthConfig, packed) -> None:
         fp.write(data.tobytes())
 
 
-# Benchmark configurations (north-star workloads).
+# Benchmark and test workloads.
 def cfg_single_dm() -> SynthConfig:
     """8-bit, 1024 ch, 65536 samples, DM-300 boxcar (width 16, +24) at sample 20000."""
     return SynthConfig(nchans=1024, nsamps=65536, nbits=8, fch1=1500.0, foff=-0.390625,

@@ -25,7 +25,7 @@ def _unpack_2bit_big(packed, torch):
 
 
 @pytest.fixture(scope="module")
-def cfg2_sweep():
+def sweep_2bit_2048dm():
     import torch
 
     from sigpyproc3_amd import _lib
@@ -44,10 +44,10 @@ def cfg2_sweep():
     torch.cuda.empty_cache()
 
 
-def test_every_row_checksum(cfg2_sweep):
+def test_every_row_checksum(sweep_2bit_2048dm):
     import torch
 
-    cfg, table, packed, plan, plane = cfg2_sweep
+    cfg, table, packed, plan, plane = sweep_2bit_2048dm
     n, nch = cfg.nsamps, cfg.nchans
     x = _unpack_2bit_big(packed, torch).reshape(n, nch)
     d = torch.from_numpy(table.astype(np.int64)).cuda()            # [ndm, nch]
@@ -70,12 +70,12 @@ def test_every_row_checksum(cfg2_sweep):
     assert int(plan.total_samples) == 2117087959
 
 
-def test_rows_vs_c_oracle_and_single_dm_kernel(cfg2_sweep):
+def test_rows_vs_c_oracle_and_single_dm_kernel(sweep_2bit_2048dm):
     import torch
 
     from sigpyproc3_amd import _lib
 
-    cfg, table, packed, plan, plane = cfg2_sweep
+    cfg, table, packed, plan, plane = sweep_2bit_2048dm
     x = _unpack_2bit_big(packed, torch).cpu().numpy()
     rows = [0, 1, 150, 1700, 2047]
     ref = c_oracle.sweep(x, cfg.nchans, table[rows], ld=plane.shape[1])
@@ -92,8 +92,8 @@ def test_rows_vs_c_oracle_and_single_dm_kernel(cfg2_sweep):
         assert torch.equal(out[:n], plane[i, :n])
 
 
-def test_injected_pulses_found(cfg2_sweep):
-    cfg, table, packed, plan, plane = cfg2_sweep
+def test_injected_pulses_found(sweep_2bit_2048dm):
+    cfg, table, packed, plan, plane = sweep_2bit_2048dm
     for p in cfg.pulses:
         i = int(p.dm)
         row = plane[i, : int(plan.row_len[i])]

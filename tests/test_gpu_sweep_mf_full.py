@@ -26,7 +26,7 @@ ROWS = (0, 1, 499, 500, 501, 777, 1023)
 
 
 @pytest.fixture(scope="module")
-def cfg4_sweep():
+def sweep_8bit_1024dm():
     import torch
 
     from sigpyproc3_amd import _lib
@@ -47,10 +47,10 @@ def cfg4_sweep():
     torch.cuda.empty_cache()
 
 
-def test_sweep_every_row_checksum(cfg4_sweep):
+def test_sweep_every_row_checksum(sweep_8bit_1024dm):
     import torch
 
-    cfg, table, packed, plan, plane, _, _ = cfg4_sweep
+    cfg, table, packed, plan, plane, _, _ = sweep_8bit_1024dm
     n, nch = cfg.nsamps, cfg.nchans
     x = packed.reshape(n, nch)
     d = torch.from_numpy(table.astype(np.int64)).cuda()
@@ -68,8 +68,8 @@ def test_sweep_every_row_checksum(cfg4_sweep):
     assert int(plan.row_len.min()) == n - int(table.max())
 
 
-def test_sweep_rows_vs_c_oracle(cfg4_sweep):
-    cfg, table, packed, plan, plane, _, _ = cfg4_sweep
+def test_sweep_rows_vs_c_oracle(sweep_8bit_1024dm):
+    cfg, table, packed, plan, plane, _, _ = sweep_8bit_1024dm
     x = packed.cpu().numpy()
     rows = list(ROWS)
     ref = c_oracle.sweep(x, cfg.nchans, table[rows], ld=plane.shape[1])
@@ -78,8 +78,8 @@ def test_sweep_rows_vs_c_oracle(cfg4_sweep):
         np.testing.assert_array_equal(plane[i, :n].cpu().numpy(), ref[k, :n], err_msg=f"row {i}")
 
 
-def test_normalise_and_mf_vs_fp64_oracle(cfg4_sweep):
-    _, _, _, plan, plane, z, (snr, peak, widx) = cfg4_sweep
+def test_normalise_and_mf_vs_fp64_oracle(sweep_8bit_1024dm):
+    _, _, _, plan, plane, z, (snr, peak, widx) = sweep_8bit_1024dm
     worst = 0.0
     for i in ROWS:
         n = int(plan.row_len[i])
@@ -96,8 +96,8 @@ def test_normalise_and_mf_vs_fp64_oracle(cfg4_sweep):
     print(f"max relative snr difference vs float64 oracle: {worst:.3e}")
 
 
-def test_injected_pulse_is_best_detection(cfg4_sweep):
-    cfg, _, _, plan, _, _, (snr, peak, widx) = cfg4_sweep
+def test_injected_pulse_is_best_detection(sweep_8bit_1024dm):
+    cfg, _, _, plan, _, _, (snr, peak, widx) = sweep_8bit_1024dm
     (p,) = cfg.pulses
     best = int(np.argmax(snr))
     assert abs(best - int(p.dm)) <= 1

@@ -74,7 +74,7 @@ def test_sweep_plan_tiles_and_lengths():
     lib.spp_sweep_plan_destroy(plan)
     np.testing.assert_array_equal(rl, (1 << 20) - table.max(axis=1))
     assert info.max_len == (1 << 20)  # DM 0
-    assert info.min_len == (1 << 20) - 29683  # max delay at DM 2047
+    assert info.min_len == (1 << 20) - 29683  # max delay at DM 2047 (2-bit, 4096 ch, 2^20 samples)
     assert info.total_samples == rl.sum() == 2117087959
     assert info.ndm_tiles == 128 and info.max_spread <= 252
 

@@ -93,7 +93,7 @@ def test_notebook_kat(datadir):
     assert tim.size == 187503
     # the notebook prints the float32 sum to 7 significant digits
     assert f"{float(tim.sum()):.6e}" == "1.963699e+07"
-    assert float(tim.sum()) == 19636992.0  # exact integer sum (reproduced)
+    assert float(tim.sum()) == 19636992.0  # exact integer sum
     assert (tim.max(), tim.min(), np.median(tim)) == (121.0, 88.0, 105.0)
 
 
@@ -120,7 +120,7 @@ def test_read_plan_blocks_tile_outputs():
 
 
 def test_read_plan_blocks_reference_example():
-    # cfg1: 6 full reads and one of 11740 samples
+    # 8-bit 1024 ch, 65536 samples, DM 300: 6 full reads and one of 11740 samples
     blocks = O.read_plan_blocks(65536, 16384, 7418)
     assert [b[1] for b in blocks] == [16384] * 6 + [11740]
 
