@@ -33,6 +33,7 @@ constexpr int MF_B = 4096;       // outputs per MF block
 constexpr int MF_THREADS = 256;
 constexpr int MF_WMAX = 1024;    // largest boxcar width supported
 constexpr int MF_MAXW = 32;      // largest number of widths
+constexpr int MF_NWREG = 16;     // widths tracked in registers (more use the general loop)
 constexpr int CT_B = 1024;       // outputs per template-convolution block
 constexpr int CT_TAPS = 2048;    // largest template length
 
@@ -276,6 +277,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_boxcar_part(
     __shared__ double Q[MF_B + MF_WMAX + 1];
     __shared__ double sa[MF_THREADS], sb[MF_THREADS];
     __shared__ double coef_mean[MF_MAXW], coef_inv[MF_MAXW];
+    __shared__ int32_t sw[MF_MAXW];
     __shared__ Best bred[MF_THREADS];
     const int64_t row = blockIdx.y;
     const MfRow r = rows[row];
@@ -296,6 +298,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_boxcar_part(
         const double w = widths[tid];
         coef_mean[tid] = w * srow / ng;
         coef_inv[tid] = 1.0 / sqrt(w * (1.0 - w / ng));
+        sw[tid] = widths[tid];
     }
     const int64_t lw = min<int64_t>(MF_B, r.n - a);
     const int nel = static_cast<int>(lw) + wmax;  // zt[a .. a + nel)
@@ -322,15 +325,47 @@ __global__ __launch_bounds__(MF_THREADS) void k_boxcar_part(
         if (k < nel) Q[k + 1] = ex + loc[e];
     }
     __syncthreads();
-    for (int u = tid; u < lw; u += MF_THREADS) {
-        for (int wi = 0; wi < nw; ++wi) {
-            const int w = widths[wi];
-            const double conv = (Q[u + w] - Q[u] - coef_mean[wi]) * coef_inv[wi];
-            const float cv = static_cast<float>(conv);
-            if (better(cv, wi, a + u, best)) {
-                best.val = cv;
+    if (nw <= MF_NWREG) {
+        // Per-width maxima in registers: t increases within a thread, so a strict > keeps
+        // each width's first maximum; the template-major tie order is applied once below.
+        float bv[MF_NWREG];
+        int32_t bt[MF_NWREG];
+#pragma unroll
+        for (int wi = 0; wi < MF_NWREG; ++wi) {
+            bv[wi] = -FLT_MAX;
+            bt[wi] = -1;
+        }
+        for (int u = tid; u < lw; u += MF_THREADS) {
+            const double q0 = Q[u];
+#pragma unroll
+            for (int wi = 0; wi < MF_NWREG; ++wi) {
+                if (wi < nw) {
+                    const float cv = static_cast<float>((Q[u + sw[wi]] - q0 - coef_mean[wi]) * coef_inv[wi]);
+                    if (cv > bv[wi]) {
+                        bv[wi] = cv;
+                        bt[wi] = u;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int wi = 0; wi < MF_NWREG; ++wi) {
+            if (wi < nw && bt[wi] >= 0 && better(bv[wi], wi, a + bt[wi], best)) {
+                best.val = bv[wi];
                 best.w = wi;
-                best.t = a + u;
+                best.t = a + bt[wi];
+            }
+        }
+    } else {
+        for (int u = tid; u < lw; u += MF_THREADS) {
+            for (int wi = 0; wi < nw; ++wi) {
+                const double conv = (Q[u + sw[wi]] - Q[u] - coef_mean[wi]) * coef_inv[wi];
+                const float cv = static_cast<float>(conv);
+                if (better(cv, wi, a + u, best)) {
+                    best.val = cv;
+                    best.w = wi;
+                    best.t = a + u;
+                }
             }
         }
     }
