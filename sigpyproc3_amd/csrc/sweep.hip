@@ -17,10 +17,12 @@
 // while the current stage is summed, then into the other half of a double-buffered LDS
 // area (one barrier per stage). Each wave owns K DM rows; per channel and DM row a lane
 // reads 4 LDS words (lanes read consecutive words: bank-conflict free) at a byte offset
-// the planner's relative delays give, and adds them into SWAR accumulators:
+// the planner's relative delays give (scalar loads, one stage's row at a time), and adds
+// them into SWAR accumulators:
 //   nibble fields (1/2-bit): groups of GN channels (GN * maxval <= 15: 5 for 2-bit) summed
-//                  with v_add3, two groups widened to byte lanes together, byte lanes to
-//                  16-bit lanes every SB stages;
+//                  with v_add3; per pair of groups the odd fields are masked into byte lanes
+//                  and the raw words summed (even = raw - 16 * odd, recovered when byte
+//                  lanes are widened to 16-bit lanes every SB stages);
 //   byte fields (4-bit): summed over the stage, then widened to 16-bit lanes;
 //   8-bit data: even/odd bytes (v_and / v_perm) go straight to 16-bit lanes;
 //   32-bit lanes only when nchans * maxval > 65535.
@@ -38,7 +40,8 @@ namespace {
 
 constexpr int SW_S = 256;                // interleave stride = words per wave row
 constexpr int SW_DT = 16;                // DM rows per tile
-constexpr int SW_LW = 496;               // LDS words per staged channel
+constexpr int SW_LW = 512;               // LDS words per staged channel (2 KiB: a multiple of the
+                                         // 256-byte unit of ds_read2st64 offsets)
 constexpr int SW_SPREAD_MAX = SW_LW - SW_S - 4;
 constexpr int B_TJ = 128;                // words per build block (time)
 constexpr int G_STRIDE = 68;             // bytes per staged row in the build kernel
@@ -150,7 +153,7 @@ __device__ __forceinline__ uint32_t odd_bytes(uint32_t x) { return __builtin_amd
 template <int NBITS, bool USE32, int K, int WAVES, int CC, int MINW>
 __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW))) void k_sweep(
     const uint32_t* __restrict__ G, int64_t gpitch, int64_t glen, int64_t nchans,
-    const int32_t* __restrict__ relw, const int32_t* __restrict__ basew,
+    const int32_t* __restrict__ relw, int64_t rpitch, const int32_t* __restrict__ basew,
     const int32_t* __restrict__ tile_dm0, const int32_t* __restrict__ tile_ndm,
     const int32_t* __restrict__ tile_spread, const int32_t* __restrict__ row_len,
     float* __restrict__ plane, int64_t ld, int64_t shrink, int64_t ntiles_dm, int64_t nblocks) {
@@ -166,7 +169,6 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
     constexpr int BUFW = CC * SW_LW;                    // words per LDS buffer
     static_assert(SW_LW % 4 == 0, "whole uint4 chunks");
     static_assert(D == SW_DT, "the planner forms tiles of SW_DT rows");
-    constexpr int NRL = (D * CC + THREADS - 1) / THREADS;  // stage offsets per thread
     static_assert(IL != 8 || (SG::NG > 0 && SG::NG % 2 == 0), "nibble groups come in pairs");
     static_assert(NBITS != 4 || CC * 15 <= 255, "4-bit stage sums must fit byte lanes");
     constexpr int NG = SG::NG, GN = SG::GN;
@@ -176,7 +178,6 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
     static_assert(SB >= 1 && SW16 >= 1, "lanes must hold one stage");
 
     __shared__ uint32_t lds[2 * BUFW];
-    __shared__ int32_t srel[2][D * CC];   // byte offset into lds of (DM row, stage channel)
 
     // XCD-aware order: consecutive blocks on one XCD walk the DM tiles of one time tile.
     const int64_t b = blockIdx.x;
@@ -216,18 +217,25 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
             for (int i = 0; i < 4 * IL; ++i) accD[k][i] = 0;
     }
 
+    // Nibble data: accB[.][.][1] holds the odd fields (byte lanes); accB[.][.][0] the plain
+    // 32-bit sum of the group words, which wraps. As integers the group words sum to
+    // even + 16 * odd, and both byte-lane vectors are below 2^32 while their lanes are at most
+    // 255, so even = raw - (odd << 4) exactly modulo 2^32.
     auto flushB = [&]() {
         if constexpr (IL == 8) {
 #pragma unroll
             for (int k = 0; k < K; ++k)
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-#pragma unroll
-                    for (int p = 0; p < NP; ++p) {
-                        accW[k][q][p][0] += accB[k][q][p] & 0x00ff00ffu;
-                        accW[k][q][p][1] += odd_bytes(accB[k][q][p]);
-                        accB[k][q][p] = 0;
-                    }
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t odd = accB[k][q][1];
+                    const uint32_t even = accB[k][q][0] - (odd << 4);
+                    accW[k][q][0][0] += even & 0x00ff00ffu;
+                    accW[k][q][0][1] += odd_bytes(even);
+                    accW[k][q][1][0] += odd & 0x00ff00ffu;
+                    accW[k][q][1][1] += odd_bytes(odd);
+                    accB[k][q][0] = 0;
+                    accB[k][q][1] = 0;
+                }
         }
     };
     // Sample index (0..IL-1) of 16-bit lane l of accW[.][.][p][h].
@@ -252,10 +260,12 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
     // Staging: wave w copies channels w, w + WAVES, ... of a stage (channel indices, bounds
     // and source rows are wave-uniform); lane l loads words 4l + 256i of each window
     // G[c][t0 + base_c .. + stage_words). Tile bases (basew) are loaded one stage before they
-    // are used; LDS offsets come from relw = d - base (t0, a multiple of 4, drops out).
+    // are used. The LDS byte offset of (DM row, channel) inside its window, 4 * (d - base)
+    // (t0, a multiple of 4, drops out), is read by each wave for its own rows from relw
+    // [row][rpitch] with scalar loads; rows are zero-padded past nchans, so the last stage's
+    // missing channels read their zero-filled windows at offset 0.
     uint4 pf[CPW][NLD];
     int32_t bw_cur[CPW], bw_nxt[CPW];
-    int32_t my_rel[NRL];
     auto load_bw = [&](int st, int32_t* bw) {
 #pragma unroll
         for (int i = 0; i < CPW; ++i) {
@@ -286,18 +296,6 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
                 }
             }
         }
-#pragma unroll
-        for (int u = 0; u < NRL; ++u) {
-            const int e = tid + THREADS * u;
-            const int dmi = e / CC, cl = e % CC;
-            int32_t rel = cl * SW_LW;  // a channel past the end reads its zero-filled window
-            const int64_t c = c0 + cl;
-            if (e < D * CC && c < nchans) {
-                const int row = dm0 + min(dmi, ndm_t - 1);
-                rel += relw[static_cast<int64_t>(row) * nchans + c];
-            }
-            my_rel[u] = rel;
-        }
     };
     auto commit = [&](int buf) {
 #pragma unroll
@@ -310,15 +308,13 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
                 if (m < stage_words) *reinterpret_cast<uint4*>(&lds[buf * BUFW + cl * SW_LW + m]) = pf[i][r];
             }
         }
-#pragma unroll
-        for (int u = 0; u < NRL; ++u) {
-            const int e = tid + THREADS * u;
-            if (e < D * CC) srel[buf][e] = 4 * (buf * BUFW + my_rel[u]);
-        }
     };
 
-    const char* __restrict__ Lb = reinterpret_cast<const char*>(lds);
-    const int lane4 = 4 * lane;
+    const char* __restrict__ Lb = reinterpret_cast<const char*>(lds) + 4 * lane;
+    const int32_t* __restrict__ rrow[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        rrow[k] = relw + static_cast<int64_t>(dm0 + min(wave * K + k, ndm_t - 1)) * rpitch;
 
     load_bw(0, bw_cur);
     load_bw(1, bw_nxt);
@@ -334,10 +330,13 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
             load_bw(st + 2, bw_nxt);
             prefetch(st + 1);
         }
-        const int32_t* __restrict__ R = srel[buf];
+        const int32_t* __restrict__ R[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) R[k] = rrow[k] + static_cast<int64_t>(st) * CC;
+        const char* __restrict__ Lbuf = Lb + 4 * buf * BUFW;
         // The 4 words (lane + 64q) of DM row k, stage channel cl.
         auto ld4 = [&](int k, int cl, uint32_t (&w)[4]) {
-            const char* p = Lb + R[(wave * K + k) * CC + cl] + lane4;
+            const char* p = Lbuf + 4 * cl * SW_LW + R[k][cl];
 #pragma unroll
             for (int q = 0; q < 4; ++q) w[q] = *reinterpret_cast<const uint32_t*>(p + 256 * q);
         };
@@ -362,7 +361,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
                     group_sum(k, (g + 1) * GN, GN, bb);
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        accB[k][q][0] += (a[q] & 0x0f0f0f0fu) + (bb[q] & 0x0f0f0f0fu);
+                        accB[k][q][0] += a[q] + bb[q];
                         accB[k][q][1] += ((a[q] >> 4) & 0x0f0f0f0fu) + ((bb[q] >> 4) & 0x0f0f0f0fu);
                     }
                     // Keep the scheduler from hoisting every row's LDS reads (register spills).
@@ -454,7 +453,8 @@ struct spp_sweep_plan {
     bool use32 = false;
     int variant = 0;  // k_sweep block shape (launch_sweep)
     int64_t gpitch = 0;
-    std::vector<int32_t> relw;   // [ndm][nchans] = delay - (tile base & ~3): LDS word offset
+    std::vector<int32_t> relw;   // [ndm][rpitch] = 4 * (delay - tile base): LDS byte offset
+    int64_t rpitch = 0;          // nchans + 32 rounded up to 16; zero past nchans
     int64_t off_relw = 0, off_base = 0, off_dm0 = 0, off_ndm = 0, off_spread = 0, off_rowlen = 0,
             off_g = 0, ws_bytes = 0;
     const void* uploaded_to = nullptr;
@@ -510,7 +510,8 @@ extern "C" int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int
         p->total += len;
     }
     // Greedy DM tiles: consecutive rows while the per-channel delay spread fits LDS.
-    p->relw.resize(static_cast<size_t>(ndm * nchans));
+    p->rpitch = ((nchans + 32 + 15) / 16) * 16;
+    p->relw.assign(static_cast<size_t>(ndm * p->rpitch), 0);
     std::vector<int32_t> lo(nchans), hi(nchans);
     for (int64_t i = 0; i < ndm;) {
         const int32_t* r0 = &p->table[i * nchans];
@@ -536,7 +537,7 @@ extern "C" int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int
         p->tile_spread.push_back(spread);
         for (int64_t r = i; r < j; ++r)
             for (int64_t c = 0; c < nchans; ++c)
-                p->relw[r * nchans + c] = p->table[r * nchans + c] - (lo[c] & ~int32_t(3));
+                p->relw[r * p->rpitch + c] = 4 * (p->table[r * nchans + c] - (lo[c] & ~int32_t(3)));
         for (int64_t c = 0; c < nchans; ++c) p->base.push_back(lo[c] & ~int32_t(3));
         p->max_spread = std::max(p->max_spread, spread);
         i = j;
@@ -546,7 +547,7 @@ extern "C" int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int
     p->gpitch = ((nsamps + 63) / 64) * 64;
     const int64_t ntiles = static_cast<int64_t>(p->tile_dm0.size());
     int64_t off = 0;
-    p->off_relw = off; off = align256(off + 4 * ndm * nchans);
+    p->off_relw = off; off = align256(off + 4 * ndm * p->rpitch);
     p->off_base = off; off = align256(off + 4 * ntiles * nchans);
     p->off_dm0 = off; off = align256(off + 4 * ntiles);
     p->off_ndm = off; off = align256(off + 4 * ntiles);
@@ -603,7 +604,7 @@ extern "C" int spp_sweep_plan_upload(spp_sweep_plan* p, void* workspace, int64_t
     hipStream_t st = spp::as_stream(stream);
     char* ws = static_cast<char*>(workspace);
     const int64_t ntiles = static_cast<int64_t>(p->tile_dm0.size());
-    SPP_HIP(hipMemcpyAsync(ws + p->off_relw, p->relw.data(), 4 * p->ndm * p->nchans,
+    SPP_HIP(hipMemcpyAsync(ws + p->off_relw, p->relw.data(), 4 * p->ndm * p->rpitch,
                            hipMemcpyHostToDevice, st));
     SPP_HIP(hipMemcpyAsync(ws + p->off_base, p->base.data(), 4 * ntiles * p->nchans,
                            hipMemcpyHostToDevice, st));
@@ -657,7 +658,7 @@ int launch_sweep_v(const spp_sweep_plan* p, const Extent& e, char* ws, float* pl
     hipLaunchKernelGGL((k_sweep<NBITS, USE32, K, WAVES, CC, MINW>), dim3(static_cast<unsigned>(grid)),
                        dim3(WAVES * 64), 0, st, reinterpret_cast<const uint32_t*>(ws + p->off_g),
                        p->gpitch, e.glen, p->nchans, reinterpret_cast<const int32_t*>(ws + p->off_relw),
-                       reinterpret_cast<const int32_t*>(ws + p->off_base),
+                       p->rpitch, reinterpret_cast<const int32_t*>(ws + p->off_base),
                        reinterpret_cast<const int32_t*>(ws + p->off_dm0),
                        reinterpret_cast<const int32_t*>(ws + p->off_ndm),
                        reinterpret_cast<const int32_t*>(ws + p->off_spread),
