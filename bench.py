@@ -196,7 +196,10 @@ def bench_mf(args) -> int:
     if args.variant >= 0:
         plan.variant = args.variant
     else:
+        from sigpyproc3_amd import tune
+
         plan.tune(packed, plane, cache=False)
+        log(f"mf: autotune: variant {plan.variant}; k_sweep ms per variant {tune.last_times()}")
     for _ in range(args.warmup):
         plan.execute(packed, plane)
         boxcar_mf(running_normalise(plane, plan.row_len, 4096), plan.row_len, DEFAULT_WIDTHS)

@@ -42,7 +42,7 @@ constexpr int SW_S = 256;                // interleave stride = words per wave r
 constexpr int SW_DT = 16;                // DM rows per tile
 constexpr int SW_LW = 512;               // LDS words per staged channel (2 KiB: a multiple of the
                                          // 256-byte unit of ds_read2st64 offsets)
-constexpr int SW_SPREAD_MAX = SW_LW - SW_S - 4;
+constexpr int SW_SPREAD_MAX = SW_LW - SW_S - 4;  // per-channel delay spread of a DM tile
 constexpr int B_TJ = 128;                // words per build block (time)
 constexpr int G_STRIDE = 68;             // bytes per staged row in the build kernel
 
@@ -153,7 +153,7 @@ __device__ __forceinline__ uint32_t odd_bytes(uint32_t x) { return __builtin_amd
 template <int NBITS, bool USE32, int K, int WAVES, int CC, int MINW>
 __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW))) void k_sweep(
     const uint32_t* __restrict__ G, int64_t gpitch, int64_t glen, int64_t nchans,
-    const int32_t* __restrict__ relw, int64_t rpitch, const int32_t* __restrict__ basew,
+    const uint32_t* __restrict__ relw4, int64_t rpitch4, const int32_t* __restrict__ basew,
     const int32_t* __restrict__ tile_dm0, const int32_t* __restrict__ tile_ndm,
     const int32_t* __restrict__ tile_spread, const int32_t* __restrict__ row_len,
     float* __restrict__ plane, int64_t ld, int64_t shrink, int64_t ntiles_dm, int64_t nblocks) {
@@ -166,9 +166,12 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
     constexpr int T = IL * SW_S;
     constexpr int CPW = (CC + WAVES - 1) / WAVES;      // channels each wave stages per stage
     constexpr int NLD = (SW_LW + 255) / 256;            // uint4 loads per lane per staged channel
-    constexpr int BUFW = CC * SW_LW;                    // words per LDS buffer
+    constexpr int CHW = SW_LW;                          // words per staged channel window
+    constexpr int BUFW = CC * CHW;                      // words per LDS buffer
     static_assert(SW_LW % 4 == 0, "whole uint4 chunks");
     static_assert(D == SW_DT, "the planner forms tiles of SW_DT rows");
+    static_assert(CC % 4 == 0 && CC <= 32, "relw rows hold 4 offsets per word, padded for two stages");
+    constexpr int NPK = CC / 4;                         // packed offset words per row and stage
     static_assert(IL != 8 || (SG::NG > 0 && SG::NG % 2 == 0), "nibble groups come in pairs");
     static_assert(NBITS != 4 || CC * 15 <= 255, "4-bit stage sums must fit byte lanes");
     constexpr int NG = SG::NG, GN = SG::GN;
@@ -260,10 +263,11 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
     // Staging: wave w copies channels w, w + WAVES, ... of a stage (channel indices, bounds
     // and source rows are wave-uniform); lane l loads words 4l + 256i of each window
     // G[c][t0 + base_c .. + stage_words). Tile bases (basew) are loaded one stage before they
-    // are used. The LDS byte offset of (DM row, channel) inside its window, 4 * (d - base)
-    // (t0, a multiple of 4, drops out), is read by each wave for its own rows from relw
-    // [row][rpitch] with scalar loads; rows are zero-padded past nchans, so the last stage's
-    // missing channels read their zero-filled windows at offset 0.
+    // are used. The offset of (DM row, channel) inside its window, d - base (t0, a multiple of
+    // 4, drops out; at most SW_SPREAD_MAX + 3 < 256), is one byte of relw4 [row][rpitch4 words];
+    // each wave reads its own rows' bytes with scalar loads one stage ahead. Rows are
+    // zero-padded past nchans (by more than two stages of at most 32 channels), so the last
+    // stage's missing channels read their zero-filled windows at offset 0.
     uint4 pf[CPW][NLD];
     int32_t bw_cur[CPW], bw_nxt[CPW];
     auto load_bw = [&](int st, int32_t* bw) {
@@ -305,16 +309,20 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
 #pragma unroll
             for (int r = 0; r < NLD; ++r) {
                 const int m = 4 * lane + 256 * r;
-                if (m < stage_words) *reinterpret_cast<uint4*>(&lds[buf * BUFW + cl * SW_LW + m]) = pf[i][r];
+                if (m < stage_words) *reinterpret_cast<uint4*>(&lds[buf * BUFW + cl * CHW + m]) = pf[i][r];
             }
         }
     };
 
     const char* __restrict__ Lb = reinterpret_cast<const char*>(lds) + 4 * lane;
-    const int32_t* __restrict__ rrow[K];
+    const uint32_t* __restrict__ rrow[K];
+    uint32_t P[K][NPK];  // this stage's packed offsets of the wave's rows
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-        rrow[k] = relw + static_cast<int64_t>(dm0 + min(wave * K + k, ndm_t - 1)) * rpitch;
+    for (int k = 0; k < K; ++k) {
+        rrow[k] = relw4 + static_cast<int64_t>(dm0 + min(wave * K + k, ndm_t - 1)) * rpitch4;
+#pragma unroll
+        for (int u = 0; u < NPK; ++u) P[k][u] = rrow[k][u];
+    }
 
     load_bw(0, bw_cur);
     load_bw(1, bw_nxt);
@@ -330,25 +338,33 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
             load_bw(st + 2, bw_nxt);
             prefetch(st + 1);
         }
-        const int32_t* __restrict__ R[K];
+        // The next stage's offsets, waited for at the next barrier (one stage past the last
+        // reads the zero padding).
+        uint32_t Pn[K][NPK];
 #pragma unroll
-        for (int k = 0; k < K; ++k) R[k] = rrow[k] + static_cast<int64_t>(st) * CC;
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int u = 0; u < NPK; ++u) Pn[k][u] = rrow[k][static_cast<int64_t>(st + 1) * NPK + u];
+        auto rel = [&](int k, int cl) -> uint32_t { return (P[k][cl >> 2] >> (8 * (cl & 3))) & 0xffu; };
         const char* __restrict__ Lbuf = Lb + 4 * buf * BUFW;
         // The 4 words (lane + 64q) of DM row k, stage channel cl.
         auto ld4 = [&](int k, int cl, uint32_t (&w)[4]) {
-            const char* p = Lbuf + 4 * cl * SW_LW + R[k][cl];
+            const char* p = Lbuf + 4 * cl * CHW + 4 * rel(k, cl);
 #pragma unroll
             for (int q = 0; q < 4; ++q) w[q] = *reinterpret_cast<const uint32_t*>(p + 256 * q);
         };
-        // Sum of channels [c0, c0 + n) of DM row k, per word.
+        // Sum of channels [c0, c0 + n) of DM row k, per word. Loops over channels are fully
+        // unrolled (offsets are per-channel scalars); long sums are cut into runs of 4
+        // channels the scheduler may not interleave, which bounds the reads in flight.
         auto group_sum = [&](int k, int c0, int n, uint32_t (&s)[4]) {
             ld4(k, c0, s);
-#pragma unroll 4
+#pragma unroll
             for (int j = 1; j < n; ++j) {
                 uint32_t w[4];
                 ld4(k, c0 + j, w);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) s[q] += w[q];
+                if (n > 5 && j % 4 == 3) __builtin_amdgcn_sched_barrier(0);
             }
         };
         if constexpr (IL == 8) {
@@ -367,31 +383,49 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
                     // Keep the scheduler from hoisting every row's LDS reads (register spills).
                     __builtin_amdgcn_sched_barrier(0);
                 }
-        } else if constexpr (NBITS == 4) {
+        } else {
+            // Byte fields: a runtime loop over the stage's packed offset words, 4 channels per
+            // trip (short live ranges). The words are consumed from a shift register so every
+            // index stays static.
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                uint32_t a[4];
-                group_sum(k, 0, CC, a);
+                uint32_t pk[NPK];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    accW[k][q][0][0] += a[q] & 0x00ff00ffu;
-                    accW[k][q][0][1] += odd_bytes(a[q]);
-                }
-            }
-        } else {
+                for (int u = 0; u < NPK; ++u) pk[u] = P[k][u];
+                uint32_t a[4] = {0, 0, 0, 0};
+#pragma unroll 1
+                for (int u = 0; u < NPK; ++u) {
+                    const uint32_t word = pk[0];
 #pragma unroll
-            for (int k = 0; k < K; ++k)
-#pragma unroll 2
-                for (int cl = 0; cl < CC; cl += 2) {
-                    uint32_t w0[4], w1[4];
-                    ld4(k, cl, w0);
-                    ld4(k, cl + 1, w1);
+                    for (int v = 0; v + 1 < NPK; ++v) pk[v] = pk[v + 1];
+                    const char* base = Lbuf + 16 * u * CHW;
+                    uint32_t w[4][4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const char* pj = base + 4 * j * CHW + 4 * ((word >> (8 * j)) & 0xffu);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) w[j][q] = *reinterpret_cast<const uint32_t*>(pj + 256 * q);
+                    }
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        accW[k][q][0][0] += (w0[q] & 0x00ff00ffu) + (w1[q] & 0x00ff00ffu);
-                        accW[k][q][0][1] += odd_bytes(w0[q]) + odd_bytes(w1[q]);
+                        if constexpr (NBITS == 4) {
+                            a[q] += w[0][q] + w[1][q] + w[2][q] + w[3][q];  // byte lanes, <= CC * 15
+                        } else {
+                            accW[k][q][0][0] += (w[0][q] & 0x00ff00ffu) + (w[1][q] & 0x00ff00ffu);
+                            accW[k][q][0][0] += (w[2][q] & 0x00ff00ffu) + (w[3][q] & 0x00ff00ffu);
+                            accW[k][q][0][1] += odd_bytes(w[0][q]) + odd_bytes(w[1][q]);
+                            accW[k][q][0][1] += odd_bytes(w[2][q]) + odd_bytes(w[3][q]);
+                        }
                     }
                 }
+                if constexpr (NBITS == 4) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        accW[k][q][0][0] += a[q] & 0x00ff00ffu;
+                        accW[k][q][0][1] += odd_bytes(a[q]);
+                    }
+                }
+            }
         }
         if constexpr (IL == 8) {
             if (++nb == SB) {
@@ -408,6 +442,10 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
             }
         }
         if (st + 1 < nst) commit(buf ^ 1);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int u = 0; u < NPK; ++u) P[k][u] = Pn[k][u];
         __syncthreads();
     }
     flushB();
@@ -453,8 +491,8 @@ struct spp_sweep_plan {
     bool use32 = false;
     int variant = 0;  // k_sweep block shape (launch_sweep)
     int64_t gpitch = 0;
-    std::vector<int32_t> relw;   // [ndm][rpitch] = 4 * (delay - tile base): LDS byte offset
-    int64_t rpitch = 0;          // nchans + 32 rounded up to 16; zero past nchans
+    std::vector<uint8_t> relw;   // [ndm][rpitch] = delay - tile base (rounded down to 4) < 256
+    int64_t rpitch = 0;          // bytes per row: nchans + 64 rounded up to 16; zero past nchans
     int64_t off_relw = 0, off_base = 0, off_dm0 = 0, off_ndm = 0, off_spread = 0, off_rowlen = 0,
             off_g = 0, ws_bytes = 0;
     const void* uploaded_to = nullptr;
@@ -510,8 +548,9 @@ extern "C" int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int
         p->total += len;
     }
     // Greedy DM tiles: consecutive rows while the per-channel delay spread fits LDS.
-    p->rpitch = ((nchans + 32 + 15) / 16) * 16;
+    p->rpitch = ((nchans + 64 + 15) / 16) * 16;
     p->relw.assign(static_cast<size_t>(ndm * p->rpitch), 0);
+    static_assert(SW_SPREAD_MAX + 3 < 256, "window offsets fit one byte");
     std::vector<int32_t> lo(nchans), hi(nchans);
     for (int64_t i = 0; i < ndm;) {
         const int32_t* r0 = &p->table[i * nchans];
@@ -537,17 +576,18 @@ extern "C" int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int
         p->tile_spread.push_back(spread);
         for (int64_t r = i; r < j; ++r)
             for (int64_t c = 0; c < nchans; ++c)
-                p->relw[r * p->rpitch + c] = 4 * (p->table[r * nchans + c] - (lo[c] & ~int32_t(3)));
+                p->relw[r * p->rpitch + c] =
+                    static_cast<uint8_t>(p->table[r * nchans + c] - (lo[c] & ~int32_t(3)));
         for (int64_t c = 0; c < nchans; ++c) p->base.push_back(lo[c] & ~int32_t(3));
         p->max_spread = std::max(p->max_spread, spread);
         i = j;
     }
     p->use32 = static_cast<int64_t>(nchans) * maxv > 65535;
-    p->variant = (nbits <= 2) ? 2 : 1;  // 8 waves x 2 DM rows x 16 channels (tunable per plan)
+    p->variant = 1;  // 8 waves x 2 DM rows x 20 (nibble) or 16 (byte) channels; tunable per plan
     p->gpitch = ((nsamps + 63) / 64) * 64;
     const int64_t ntiles = static_cast<int64_t>(p->tile_dm0.size());
     int64_t off = 0;
-    p->off_relw = off; off = align256(off + 4 * ndm * p->rpitch);
+    p->off_relw = off; off = align256(off + ndm * p->rpitch);
     p->off_base = off; off = align256(off + 4 * ntiles * nchans);
     p->off_dm0 = off; off = align256(off + 4 * ntiles);
     p->off_ndm = off; off = align256(off + 4 * ntiles);
@@ -604,7 +644,7 @@ extern "C" int spp_sweep_plan_upload(spp_sweep_plan* p, void* workspace, int64_t
     hipStream_t st = spp::as_stream(stream);
     char* ws = static_cast<char*>(workspace);
     const int64_t ntiles = static_cast<int64_t>(p->tile_dm0.size());
-    SPP_HIP(hipMemcpyAsync(ws + p->off_relw, p->relw.data(), 4 * p->ndm * p->rpitch,
+    SPP_HIP(hipMemcpyAsync(ws + p->off_relw, p->relw.data(), p->ndm * p->rpitch,
                            hipMemcpyHostToDevice, st));
     SPP_HIP(hipMemcpyAsync(ws + p->off_base, p->base.data(), 4 * ntiles * p->nchans,
                            hipMemcpyHostToDevice, st));
@@ -657,8 +697,8 @@ int launch_sweep_v(const spp_sweep_plan* p, const Extent& e, char* ws, float* pl
     SPP_REQUIRE(grid <= 0x7fffffffLL, "sweep grid too large");
     hipLaunchKernelGGL((k_sweep<NBITS, USE32, K, WAVES, CC, MINW>), dim3(static_cast<unsigned>(grid)),
                        dim3(WAVES * 64), 0, st, reinterpret_cast<const uint32_t*>(ws + p->off_g),
-                       p->gpitch, e.glen, p->nchans, reinterpret_cast<const int32_t*>(ws + p->off_relw),
-                       p->rpitch, reinterpret_cast<const int32_t*>(ws + p->off_base),
+                       p->gpitch, e.glen, p->nchans, reinterpret_cast<const uint32_t*>(ws + p->off_relw),
+                       p->rpitch / 4, reinterpret_cast<const int32_t*>(ws + p->off_base),
                        reinterpret_cast<const int32_t*>(ws + p->off_dm0),
                        reinterpret_cast<const int32_t*>(ws + p->off_ndm),
                        reinterpret_cast<const int32_t*>(ws + p->off_spread),
@@ -668,17 +708,18 @@ int launch_sweep_v(const spp_sweep_plan* p, const Extent& e, char* ws, float* pl
     return SPP_OK;
 }
 
-// Block shapes: (DM rows per wave, waves, channels per stage); all tile SW_DT rows.
-// Nibble data (1/2-bit): stages of 10 or 20 channels give 5-channel nibble groups for 2-bit.
+// Block shapes: (DM rows per wave, waves, channels per stage, min waves per SIMD); all tile
+// SW_DT rows and take a multiple of 4 channels per stage (one packed offset word per 4).
+// Nibble data (1/2-bit): 20 channels per stage give 5-channel nibble groups for 2-bit data.
 template <int NBITS, bool USE32>
 int launch_sweep(const spp_sweep_plan* p, const Extent& e, char* ws, float* plane, int64_t ld,
                  hipStream_t st) {
     if constexpr (NBITS <= 2) {
         switch (p->variant) {
             case 1: return launch_sweep_v<NBITS, USE32, 2, 8, 20, 4>(p, e, ws, plane, ld, st);
-            case 2: return launch_sweep_v<NBITS, USE32, 2, 8, 16, 4>(p, e, ws, plane, ld, st);
-            case 3: return launch_sweep_v<NBITS, USE32, 4, 4, 20, 2>(p, e, ws, plane, ld, st);
-            default: return launch_sweep_v<NBITS, USE32, 2, 8, 10, 4>(p, e, ws, plane, ld, st);
+            case 2: return launch_sweep_v<NBITS, USE32, 2, 8, 12, 4>(p, e, ws, plane, ld, st);
+            case 3: return launch_sweep_v<NBITS, USE32, 2, 8, 8, 4>(p, e, ws, plane, ld, st);
+            default: return launch_sweep_v<NBITS, USE32, 2, 8, 16, 4>(p, e, ws, plane, ld, st);
         }
     } else {
         switch (p->variant) {

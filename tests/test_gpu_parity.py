@@ -181,6 +181,8 @@ def _sweep_case(rng, nsamps, nchans, nbits, dms, foff, tsamp, fch1=1500.0, varia
     plan = SweepPlan(table, nsamps, nbits, 1 if order == "big" else 0)
     dev = _lib.to_device(packed)
     ref = c_oracle.sweep(samples, nchans, table, ld=plan.max_len)
+    if variants == "all":
+        variants = range(SweepPlan.variant_count())
     for v in variants:
         if v is not None:
             plan.variant = v
@@ -197,7 +199,7 @@ def _sweep_case(rng, nsamps, nchans, nbits, dms, foff, tsamp, fch1=1500.0, varia
 @pytest.mark.parametrize("nchans", [64, 336, 1000])
 def test_sweep_vs_oracle(nbits, nchans, rng):
     _sweep_case(rng, 20000 + 333, nchans, nbits, np.arange(0, 61, 1.5), -300.0 / nchans, 256e-6,
-                variants=(0, 1, 2, 3))
+                variants="all")
 
 
 def test_sweep_wide_spread_tiles(rng):
@@ -208,10 +210,10 @@ def test_sweep_wide_spread_tiles(rng):
 
 def test_sweep_wide_accumulators(rng):
     # 4-bit with > 4368 channels and 8-bit with > 257 channels use 32-bit accumulators
-    _sweep_case(rng, 4096 + 7, 4400, 4, np.array([0.0, 3.0, 7.0]), -0.05, 64e-6, variants=(0, 1, 2, 3))
-    _sweep_case(rng, 5000, 1024, 8, np.arange(0, 20.0), -0.3, 64e-6, variants=(0, 1, 2, 3))
+    _sweep_case(rng, 4096 + 7, 4400, 4, np.array([0.0, 3.0, 7.0]), -0.05, 64e-6, variants="all")
+    _sweep_case(rng, 5000, 1024, 8, np.arange(0, 20.0), -0.3, 64e-6, variants="all")
     # 2-bit above 21845 channels and 1-bit above 65535 channels overflow 16-bit lanes
-    _sweep_case(rng, 3000, 22000, 2, np.array([0.0, 1.0, 2.0]), -0.01, 64e-6, variants=(0, 1, 2, 3))
+    _sweep_case(rng, 3000, 22000, 2, np.array([0.0, 1.0, 2.0]), -0.01, 64e-6, variants="all")
     _sweep_case(rng, 2100, 65600, 1, np.array([0.0, 1.0]), -0.004, 64e-6, variants=(1, 3))
 
 
