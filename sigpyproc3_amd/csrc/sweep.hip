@@ -45,6 +45,7 @@ constexpr int SW_LW = 512;               // LDS words per staged channel (2 KiB:
 constexpr int SW_SPREAD_MAX = SW_LW - SW_S - 4;  // per-channel delay spread of a DM tile
 constexpr int B_TJ = 128;                // words per build block (time)
 constexpr int G_STRIDE = 68;             // bytes per staged row in the build kernel
+static_assert(SW_S == 2 * B_TJ, "stage-1 blocks two apart share staged rows");
 
 typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
 
@@ -57,10 +58,14 @@ struct Fields {
 
 // Stage 1. Block (bx, by): words [bx*B_TJ, bx*B_TJ + B_TJ) of the channels in row bytes
 // [by*64, by*64 + 64) (64 * 8/NBITS channels). The rows j0 + k*S + i (k < IL, i < B_TJ) are
-// staged in LDS; thread (quad q, byte column bc) then builds words j0 + 4q .. 4q+3 of every
-// channel packed in byte column bc from one LDS byte per (row, word) and stores them as
-// one 16-byte write per channel (a wave writes 512 contiguous bytes per channel). Only the
-// first glen words of each channel row are written (the sweep reads no further).
+// staged in LDS. Blocks are numbered with by fastest and consecutive numbers on one XCD, so
+// the column blocks of a row range (which share its 128-byte lines) and the blocks bx,
+// bx + 2, ..., bx + 2(IL - 1) (which stage the same rows, S being 2 * B_TJ) run close together
+// and re-read those rows from that XCD's L2. Thread (quad q, byte column bc) then builds
+// words j0 + 4q .. 4q+3 of every channel packed in byte column bc from one LDS byte per
+// (row, word) and stores them as one 16-byte write per channel (a wave writes 512 contiguous
+// bytes per channel). Only the first glen words of each channel row are written (the sweep
+// reads no further).
 template <int NBITS, bool BIG, bool ALIGNED>
 __global__ __launch_bounds__(256) void k_build_g(const uint8_t* __restrict__ in, int64_t nsamps,
                                                  int64_t nchans, uint32_t* __restrict__ G,
@@ -71,9 +76,13 @@ __global__ __launch_bounds__(256) void k_build_g(const uint8_t* __restrict__ in,
     constexpr int ROWS = IL * B_TJ;
     __shared__ uint8_t raw[ROWS * G_STRIDE];
     const int tid = threadIdx.x;
-    const int64_t j0 = static_cast<int64_t>(blockIdx.x) * B_TJ;
     const int64_t row_bytes = nchans * NBITS / 8;
-    const int64_t b0 = static_cast<int64_t>(blockIdx.y) * 64;
+    const int64_t nby = (row_bytes + 63) / 64;
+    const int64_t b = blockIdx.x;
+    const int64_t logical = (b % 8) * (gridDim.x / 8) + b / 8;
+    if (logical >= nby * ((glen + B_TJ - 1) / B_TJ)) return;
+    const int64_t j0 = (logical / nby) * B_TJ;
+    const int64_t b0 = (logical % nby) * 64;
     // Stage: element e = (row r, dword q of the row's 64 bytes); 4 byte loads per thread.
     for (int e = tid; e < ROWS * 16; e += 256) {
         const int r = e >> 4, q = e & 15;
@@ -674,9 +683,10 @@ Extent extent_of(const spp_sweep_plan* p, int64_t n) {
 template <int NBITS, bool BIG>
 int launch_build(const uint8_t* packed, const spp_sweep_plan* p, const Extent& e, uint32_t* G,
                  hipStream_t st) {
-    dim3 grid(static_cast<unsigned>(spp::ceil_div(e.glen, int64_t(B_TJ))),
-              static_cast<unsigned>(spp::ceil_div(p->nchans * NBITS / 8, int64_t(64))));
     const int64_t row_bytes = p->nchans * NBITS / 8;
+    const int64_t nblocks = spp::ceil_div(e.glen, int64_t(B_TJ)) * spp::ceil_div(row_bytes, int64_t(64));
+    SPP_REQUIRE(nblocks + 8 <= 0x7fffffffLL, "stage-1 grid too large");
+    const dim3 grid(static_cast<unsigned>(spp::ceil_div(nblocks, int64_t(8)) * 8));
     if (row_bytes % 4 == 0 && reinterpret_cast<uintptr_t>(packed) % 4 == 0)
         hipLaunchKernelGGL((k_build_g<NBITS, BIG, true>), grid, dim3(256), 0, st, packed, e.n, p->nchans, G,
                            p->gpitch, e.glen);
