@@ -44,6 +44,7 @@ constexpr int SW_LW = 512;               // LDS words per staged channel (2 KiB:
                                          // 256-byte unit of ds_read2st64 offsets)
 constexpr int SW_SPREAD_MAX = SW_LW - SW_S - 4;  // per-channel delay spread of a DM tile
 constexpr int B_TJ = 128;                // words per build block (time)
+constexpr int B_THREADS = 512;           // threads per build block
 constexpr int G_STRIDE = 68;             // bytes per staged row in the build kernel
 static_assert(SW_S == 2 * B_TJ, "stage-1 blocks two apart share staged rows");
 
@@ -67,14 +68,21 @@ struct Fields {
 // bytes per channel). Only the first glen words of each channel row are written (the sweep
 // reads no further).
 template <int NBITS, bool BIG, bool ALIGNED>
-__global__ __launch_bounds__(256) void k_build_g(const uint8_t* __restrict__ in, int64_t nsamps,
-                                                 int64_t nchans, uint32_t* __restrict__ G,
-                                                 int64_t gpitch, int64_t glen) {
+__global__ __launch_bounds__(B_THREADS) void k_build_g(const uint8_t* __restrict__ in, int64_t nsamps,
+                                                       int64_t nchans, uint32_t* __restrict__ G,
+                                                       int64_t gpitch, int64_t glen) {
     using F = Fields<NBITS>;
     constexpr int IL = F::IL;
     constexpr int PER = (NBITS < 8) ? 8 / NBITS : 1;     // channels per byte
     constexpr int ROWS = IL * B_TJ;
-    __shared__ uint8_t raw[ROWS * G_STRIDE];
+    // Staged row r sits at position (r % 4) * RQ + r / 4: the rows 4qd + u a build step reads
+    // for one u are then consecutive (one bank each), and the 4 rows a staging wave writes
+    // fall on different banks.
+    constexpr int RQ = ROWS / 4 + 1;
+    constexpr int NE = ROWS * 16 / B_THREADS;             // staged dwords per thread
+    constexpr int BATCH = NE < 32 ? NE : 32;              // loads in flight per thread
+    static_assert(NE % BATCH == 0, "whole batches");
+    __shared__ uint8_t raw[4 * RQ * G_STRIDE];
     const int tid = threadIdx.x;
     const int64_t row_bytes = nchans * NBITS / 8;
     const int64_t nby = (row_bytes + 63) / 64;
@@ -83,30 +91,41 @@ __global__ __launch_bounds__(256) void k_build_g(const uint8_t* __restrict__ in,
     if (logical >= nby * ((glen + B_TJ - 1) / B_TJ)) return;
     const int64_t j0 = (logical / nby) * B_TJ;
     const int64_t b0 = (logical % nby) * 64;
-    // Stage: element e = (row r, dword q of the row's 64 bytes); 4 byte loads per thread.
-    for (int e = tid; e < ROWS * 16; e += 256) {
-        const int r = e >> 4, q = e & 15;
-        const int64_t row = j0 + static_cast<int64_t>(r / B_TJ) * SW_S + (r % B_TJ);
-        uint32_t v = 0;
-        if (row < nsamps) {
-            const uint8_t* src = in + row * row_bytes + b0 + 4 * q;
-            const int64_t left = row_bytes - (b0 + 4 * q);
-            if (ALIGNED) {
-                // row_bytes and the stream start are multiples of 4: whole dwords or none
-                if (left > 0) v = *reinterpret_cast<const uint32_t*>(src);
-            } else {
+    // Stage: element e = (row r, dword q of the row's 64 bytes), loaded BATCH at a time.
+#pragma unroll 1
+    for (int e0 = 0; e0 < NE; e0 += BATCH) {
+        uint32_t v[BATCH];
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    if (u < left) v |= static_cast<uint32_t>(src[u]) << (8 * u);
+        for (int i = 0; i < BATCH; ++i) {
+            const int e = tid + B_THREADS * (e0 + i);
+            const int r = e >> 4, q = e & 15;
+            const int64_t row = j0 + static_cast<int64_t>(r / B_TJ) * SW_S + (r % B_TJ);
+            v[i] = 0;
+            if (row < nsamps) {
+                const uint8_t* src = in + row * row_bytes + b0 + 4 * q;
+                const int64_t left = row_bytes - (b0 + 4 * q);
+                if (ALIGNED) {
+                    // row_bytes and the stream start are multiples of 4: whole dwords or none
+                    if (left > 0) v[i] = *reinterpret_cast<const uint32_t*>(src);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (u < left) v[i] |= static_cast<uint32_t>(src[u]) << (8 * u);
+                }
             }
         }
-        *reinterpret_cast<uint32_t*>(&raw[r * G_STRIDE + 4 * q]) = v;
+#pragma unroll
+        for (int i = 0; i < BATCH; ++i) {
+            const int e = tid + B_THREADS * (e0 + i);
+            const int r = e >> 4, q = e & 15;
+            *reinterpret_cast<uint32_t*>(&raw[((r & 3) * RQ + (r >> 2)) * G_STRIDE + 4 * q]) = v[i];
+        }
     }
     __syncthreads();
     const int qd = tid & 31;
     const int64_t j = j0 + 4 * qd;
     if (j >= glen) return;
-    for (int bc = tid >> 5; bc < 64; bc += 8) {
+    for (int bc = tid >> 5; bc < 64; bc += B_THREADS / 32) {
         uint32_t w[PER][4];
 #pragma unroll
         for (int p = 0; p < PER; ++p)
@@ -116,7 +135,8 @@ __global__ __launch_bounds__(256) void k_build_g(const uint8_t* __restrict__ in,
         for (int k = 0; k < IL; ++k)
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const uint32_t byte = raw[(k * B_TJ + 4 * qd + u) * G_STRIDE + bc];
+                // row k * B_TJ + 4qd + u
+                const uint32_t byte = raw[(u * RQ + k * (B_TJ / 4) + qd) * G_STRIDE + bc];
 #pragma unroll
                 for (int p = 0; p < PER; ++p) {
                     const int shift = (NBITS == 8) ? 0 : (BIG ? (8 - (p + 1) * NBITS) : p * NBITS);
@@ -688,10 +708,10 @@ int launch_build(const uint8_t* packed, const spp_sweep_plan* p, const Extent& e
     SPP_REQUIRE(nblocks + 8 <= 0x7fffffffLL, "stage-1 grid too large");
     const dim3 grid(static_cast<unsigned>(spp::ceil_div(nblocks, int64_t(8)) * 8));
     if (row_bytes % 4 == 0 && reinterpret_cast<uintptr_t>(packed) % 4 == 0)
-        hipLaunchKernelGGL((k_build_g<NBITS, BIG, true>), grid, dim3(256), 0, st, packed, e.n, p->nchans, G,
+        hipLaunchKernelGGL((k_build_g<NBITS, BIG, true>), grid, dim3(B_THREADS), 0, st, packed, e.n, p->nchans, G,
                            p->gpitch, e.glen);
     else
-        hipLaunchKernelGGL((k_build_g<NBITS, BIG, false>), grid, dim3(256), 0, st, packed, e.n, p->nchans, G,
+        hipLaunchKernelGGL((k_build_g<NBITS, BIG, false>), grid, dim3(B_THREADS), 0, st, packed, e.n, p->nchans, G,
                            p->gpitch, e.glen);
     SPP_LAUNCHED("k_build_g");
     return SPP_OK;
