@@ -6,12 +6,14 @@
 //
 // Running normalisation (one DM series x of length n, window W):
 //   p = x padded symmetrically by (W/2, W/2 - (W even)) samples (the padding of
-//   core/stats.py:231-236), d = p - x[0], P(k) = sum_{i<k} d[i] (float64, likewise P2
-//   for d^2). For output t: S1 = P(t+W) - P(t), S2 = P2(t+W) - P2(t),
+//   core/stats.py:231-236), d = p - x[0]. For output t the window sums are
+//   S1(t) = sum_{k=t}^{t+W-1} d[k] and S2(t) likewise for d^2 (float64),
 //   m = S1/W, v = max(S2/W - m*m, 0), s = sqrt(v) (s == 0 -> 1),
 //   z[t] = float32((x[t] - (m + x[0])) / s).
-// P is formed by segment totals (k_seg_totals), a per-row exclusive scan of the totals
-// (k_seg_scan), and a per-block local scan (k_norm), so each output reads O(1) data.
+// One block covers RN_CHB outputs of a row: it sums the first window, then slides it,
+// S(t + 1) = S(t) + d[t + W] - d[t], one block-wide scan of those differences per 2048
+// outputs. For integer-valued series (the sweep's planes) every partial sum is an exact
+// float64 integer.
 //
 // Boxcar MF (core/kernels.py:861-906 closed form, row length n, Ng = good size >= n):
 //   zt[i] = z[(i mod Ng) mod n], S = sum_{i<Ng} zt[i],
@@ -29,11 +31,12 @@ namespace {
 constexpr int RN_SEG = 2048;     // padded samples per segment / outputs per k_norm block
 constexpr int RN_THREADS = 256;
 constexpr int RN_PER = RN_SEG / RN_THREADS;  // 8
-constexpr int MF_B = 4096;       // outputs per MF block
+constexpr int RN_CHB = 16 * RN_SEG;  // outputs per normalisation block
+constexpr int MF_B = 2048;       // outputs per MF block
 constexpr int MF_THREADS = 256;
 constexpr int MF_WMAX = 1024;    // largest boxcar width supported
 constexpr int MF_MAXW = 32;      // largest number of widths
-constexpr int MF_NWREG = 16;     // widths tracked in registers (more use the general loop)
+constexpr int MF_NSP = 32;       // partial sums per row for the circular total
 constexpr int CT_B = 1024;       // outputs per template-convolution block
 constexpr int CT_TAPS = 2048;    // largest template length
 
@@ -79,6 +82,38 @@ __device__ __forceinline__ double wave_incl_scan(double v) {
     return v;
 }
 
+// Exclusive block scan of two doubles per thread, also returning the block totals (ta, tb):
+// within each wave by shuffles, then the waves' totals from LDS in wave order. Two barriers;
+// sa/sb hold one entry per wave.
+__device__ void block_exscan2_total(double& a, double& b, double& ta, double& tb, double* sa, double* sb) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double ia = wave_incl_scan(a), ib = wave_incl_scan(b);
+    double xa = __shfl_up(ia, 1, 64), xb = __shfl_up(ib, 1, 64);
+    if (lane == 0) {
+        xa = 0.0;
+        xb = 0.0;
+    }
+    if (lane == 63) {
+        sa[wave] = ia;
+        sb[wave] = ib;
+    }
+    __syncthreads();
+    double pa = 0.0, pb = 0.0, qa = 0.0, qb = 0.0;
+    for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) {
+        if (w < wave) {
+            pa += sa[w];
+            pb += sb[w];
+        }
+        qa += sa[w];
+        qb += sb[w];
+    }
+    __syncthreads();
+    a = pa + xa;
+    b = pb + xb;
+    ta = qa;
+    tb = qb;
+}
+
 // Exclusive block scan of two doubles per thread: within each wave by shuffles, then the
 // preceding waves' totals from LDS. Two barriers; sa/sb hold one entry per wave.
 __device__ void block_exscan2(double& a, double& b, double* sa, double* sb) {
@@ -107,132 +142,113 @@ __device__ void block_exscan2(double& a, double& b, double* sa, double* sb) {
 // ---------------------------------------------------------------- running normalise
 
 struct RowDesc {
-    int64_t n;     // row length
-    int64_t nseg;  // segments of the padded row
+    int64_t n;  // row length
 };
 
-// Segment totals of d = p - x[0] and d^2 over padded indices [s*SEG, (s+1)*SEG).
-__global__ __launch_bounds__(RN_THREADS) void k_seg_totals(const float* __restrict__ x, int64_t ldx,
-                                                           const RowDesc* __restrict__ rows,
-                                                           int32_t window, int64_t seg_pitch,
-                                                           double* __restrict__ tot) {
-    __shared__ double red[RN_THREADS];
-    const int64_t row = blockIdx.y;
-    const RowDesc rd = rows[row];
-    const int64_t s = blockIdx.x;
-    if (s >= rd.nseg) return;
+// z for outputs [blk * RN_CHB, ...) of one row (blocks numbered so that one XCD takes
+// consecutive blocks of a row, which share the window's samples in its L2).
+__global__ __launch_bounds__(RN_THREADS) void k_norm(const float* __restrict__ x, int64_t ldx,
+                                                     const RowDesc* __restrict__ rows, int64_t nb_row,
+                                                     int64_t nblocks, int32_t window,
+                                                     float* __restrict__ z, int64_t ldz) {
+    __shared__ double sa[RN_THREADS / 64], sb[RN_THREADS / 64], red[RN_THREADS / 64];
+    const int64_t b = blockIdx.x;
+    const int64_t logical = (b % 8) * (gridDim.x / 8) + b / 8;
+    if (logical >= nblocks) return;
+    const int64_t row = logical / nb_row;
+    const int64_t n = rows[row].n;
+    const int64_t t0 = (logical % nb_row) * RN_CHB;
+    if (t0 >= n) return;
+    const int64_t tend = min<int64_t>(t0 + RN_CHB, n);
+    const int tid = threadIdx.x;
     const float* __restrict__ xr = x + row * ldx;
+    float* __restrict__ zr = z + row * ldz;
     const int64_t L = window / 2;
-    const int64_t np = rd.n + window - 1;
     const double ref = xr[0];
+    auto dpad = [&](int64_t k) { return static_cast<double>(xr[sym_index(k - L, n)]) - ref; };
+    // A chunk's samples for the thread's outputs t = u0 + RN_PER*tid + j: p[t], p[t + W] and
+    // x[t]; the next chunk's are loaded while the current one is scanned.
+    float c0[RN_PER], cn[RN_PER], cx[RN_PER];
+    auto load_chunk = [&](int64_t u0, float (&v0)[RN_PER], float (&vn)[RN_PER], float (&vx)[RN_PER]) {
+        if (u0 >= L && u0 + RN_SEG + window - L <= n && u0 + RN_SEG <= tend) {
+            // interior chunk (almost all of them): no reflection, no bounds tests
+            const float* __restrict__ q = xr + u0 + RN_PER * tid;
+#pragma unroll
+            for (int j = 0; j < RN_PER; ++j) {
+                v0[j] = q[j - L];
+                vn[j] = q[j + window - L];
+                vx[j] = q[j];
+            }
+            return;
+        }
+#pragma unroll
+        for (int j = 0; j < RN_PER; ++j) {
+            const int64_t t = u0 + RN_PER * tid + j;
+            v0[j] = vn[j] = vx[j] = 0.f;
+            if (t < tend) {
+                v0[j] = xr[sym_index(t - L, n)];
+                vn[j] = xr[sym_index(t + window - L, n)];
+                vx[j] = xr[t];
+            }
+        }
+    };
+    load_chunk(t0, c0, cn, cx);
+    // Window sums at t0: padded indices [t0, t0 + W).
     double s1 = 0.0, s2 = 0.0;
-    for (int i = threadIdx.x; i < RN_SEG; i += RN_THREADS) {
-        const int64_t k = s * RN_SEG + i;
-        if (k < np) {
-            const double d = static_cast<double>(xr[sym_index(k - L, rd.n)]) - ref;
+    if (t0 >= L && t0 + window - L <= n) {
+        for (int64_t i = tid; i < window; i += RN_THREADS) {
+            const double d = static_cast<double>(xr[t0 + i - L]) - ref;
+            s1 += d;
+            s2 += d * d;
+        }
+    } else {
+        for (int64_t i = tid; i < window; i += RN_THREADS) {
+            const double d = dpad(t0 + i);
             s1 += d;
             s2 += d * d;
         }
     }
-    s1 = block_sum(s1, red);
-    s2 = block_sum(s2, red);
-    if (threadIdx.x == 0) {
-        tot[(row * seg_pitch + s) * 2 + 0] = s1;
-        tot[(row * seg_pitch + s) * 2 + 1] = s2;
-    }
-}
-
-// Exclusive scan of segment totals per row (sequential per row: a few hundred adds).
-__global__ void k_seg_scan(const RowDesc* __restrict__ rows, int64_t nrows, int64_t seg_pitch,
-                           const double* __restrict__ tot, double* __restrict__ pre) {
-    const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (row >= nrows) return;
-    const int64_t nseg = rows[row].nseg;
-    double a = 0.0, b = 0.0;
-    for (int64_t s = 0; s <= nseg; ++s) {
-        pre[(row * (seg_pitch + 1) + s) * 2 + 0] = a;
-        pre[(row * (seg_pitch + 1) + s) * 2 + 1] = b;
-        if (s < nseg) {
-            a += tot[(row * seg_pitch + s) * 2 + 0];
-            b += tot[(row * seg_pitch + s) * 2 + 1];
-        }
-    }
-}
-
-// z for outputs [c*SEG, c*SEG + SEG) of one row.
-__global__ __launch_bounds__(RN_THREADS) void k_norm(const float* __restrict__ x, int64_t ldx,
-                                                     const RowDesc* __restrict__ rows, int32_t window,
-                                                     int64_t seg_pitch, const double* __restrict__ pre,
-                                                     float* __restrict__ z, int64_t ldz) {
-    __shared__ double sa[RN_THREADS], sb[RN_THREADS], red[RN_THREADS];
-    const int64_t row = blockIdx.y;
-    const RowDesc rd = rows[row];
-    const int64_t a0 = static_cast<int64_t>(blockIdx.x) * RN_SEG;
-    if (a0 >= rd.n) return;
-    const int tid = threadIdx.x;
-    const float* __restrict__ xr = x + row * ldx;
-    const int64_t n = rd.n;
-    const int64_t L = window / 2;
-    const double ref = xr[0];
-    const double* __restrict__ pr = pre + row * (seg_pitch + 1) * 2;
-
-    // Range A: padded indices a0 .. a0 + SEG (segment aligned). Range B: a0 + W ...
-    const int64_t b0 = a0 + window;
-    const int64_t sbi = b0 / RN_SEG;
-    const int64_t boff = b0 - sbi * RN_SEG;
-    // P(b0) = pre[sbi] + sum of d over [sbi*SEG, b0).
-    double pb1 = 0.0, pb2 = 0.0;
-    for (int64_t i = tid; i < boff; i += RN_THREADS) {
-        const double d = static_cast<double>(xr[sym_index(sbi * RN_SEG + i - L, n)]) - ref;
-        pb1 += d;
-        pb2 += d * d;
-    }
-    pb1 = block_sum(pb1, red);
-    pb2 = block_sum(pb2, red);
-    pb1 += pr[sbi * 2 + 0];
-    pb2 += pr[sbi * 2 + 1];
-    const int64_t sai = a0 / RN_SEG;
-    const double pa1 = pr[sai * 2 + 0], pa2 = pr[sai * 2 + 1];
-
-    // Each thread owns outputs a0 + RN_PER*tid + j (j < RN_PER).
-    double da1[RN_PER], da2[RN_PER], db1[RN_PER], db2[RN_PER];
-    double ta1 = 0.0, ta2 = 0.0, tb1 = 0.0, tb2 = 0.0;
-    const int64_t np = n + window - 1;
-#pragma unroll
-    for (int j = 0; j < RN_PER; ++j) {
-        const int64_t ka = a0 + RN_PER * tid + j;
-        const int64_t kb = b0 + RN_PER * tid + j;
-        double d = 0.0;
-        if (ka < np) d = static_cast<double>(xr[sym_index(ka - L, n)]) - ref;
-        da1[j] = ta1;  // prefix before element j
-        da2[j] = ta2;
-        ta1 += d;
-        ta2 += d * d;
-        double e = 0.0;
-        if (kb < np) e = static_cast<double>(xr[sym_index(kb - L, n)]) - ref;
-        db1[j] = tb1;
-        db2[j] = tb2;
-        tb1 += e;
-        tb2 += e * e;
-    }
-    double ea = ta1, eb = ta2;
-    block_exscan2(ea, eb, sa, sb);
-    double fa = tb1, fb = tb2;
-    block_exscan2(fa, fb, sa, sb);
-    float* __restrict__ zr = z + row * ldz;
+    double S1 = block_sum(s1, red), S2 = block_sum(s2, red);
     const double invw = 1.0 / static_cast<double>(window);
+    for (int64_t u0 = t0; u0 < tend; u0 += RN_SEG) {
+        float f0[RN_PER] = {}, fn[RN_PER] = {}, fx[RN_PER] = {};
+        if (u0 + RN_SEG < tend) load_chunk(u0 + RN_SEG, f0, fn, fx);
+        // Thread's outputs u0 + RN_PER*tid + j: local exclusive prefix of d[t+W] - d[t]
+        // (zero past tend).
+        double p1[RN_PER], p2[RN_PER];
+        double a1 = 0.0, a2 = 0.0;
 #pragma unroll
-    for (int j = 0; j < RN_PER; ++j) {
-        const int64_t t = a0 + RN_PER * tid + j;
-        if (t < n) {
-            const double s1 = (pb1 + fa + db1[j]) - (pa1 + ea + da1[j]);
-            const double s2 = (pb2 + fb + db2[j]) - (pa2 + eb + da2[j]);
-            const double m = s1 * invw;
-            double v = s2 * invw - m * m;
-            v = v > 0.0 ? v : 0.0;
-            double sd = sqrt(v);
-            if (sd == 0.0) sd = 1.0;
-            zr[t] = static_cast<float>((static_cast<double>(xr[t]) - (m + ref)) / sd);
+        for (int j = 0; j < RN_PER; ++j) {
+            const int64_t t = u0 + RN_PER * tid + j;
+            p1[j] = a1;
+            p2[j] = a2;
+            if (t < tend) {
+                const double dn = static_cast<double>(cn[j]) - ref, d0 = static_cast<double>(c0[j]) - ref;
+                a1 += dn - d0;
+                a2 += dn * dn - d0 * d0;
+            }
+        }
+        double e1 = a1, e2 = a2, T1, T2;
+        block_exscan2_total(e1, e2, T1, T2, sa, sb);
+#pragma unroll
+        for (int j = 0; j < RN_PER; ++j) {
+            const int64_t t = u0 + RN_PER * tid + j;
+            if (t < tend) {
+                const double m = (S1 + e1 + p1[j]) * invw;
+                double v = (S2 + e2 + p2[j]) * invw - m * m;
+                v = v > 0.0 ? v : 0.0;
+                double sd = sqrt(v);
+                if (sd == 0.0) sd = 1.0;
+                zr[t] = static_cast<float>((static_cast<double>(cx[j]) - (m + ref)) / sd);
+            }
+        }
+        S1 += T1;
+        S2 += T2;
+#pragma unroll
+        for (int j = 0; j < RN_PER; ++j) {
+            c0[j] = f0[j];
+            cn[j] = fn[j];
+            cx[j] = fx[j];
         }
     }
 }
@@ -256,18 +272,22 @@ __device__ __forceinline__ bool better(float v, int32_t w, int64_t t, const Best
     return (w < b.w) || (w == b.w && t < b.t);
 }
 
-// S = sum_{i<Ng} zt[i] per row.
+// Partial sums of zt over [c * MF_SP, (c + 1) * MF_SP) of [0, Ng) per row (part c of
+// MF_NSP); k_boxcar_part adds a row's parts in order, so S = sum_{i<Ng} zt[i] is
+// deterministic.
 __global__ __launch_bounds__(MF_THREADS) void k_circ_sum(const float* __restrict__ z, int64_t ldz,
                                                          const MfRow* __restrict__ rows,
-                                                         double* __restrict__ S) {
-    __shared__ double red[MF_THREADS];
-    const int64_t row = blockIdx.x;
+                                                         double* __restrict__ Sp) {
+    __shared__ double red[MF_THREADS / 64];
+    const int64_t row = blockIdx.y;
     const MfRow r = rows[row];
     const float* __restrict__ zr = z + row * ldz;
+    const int64_t per = (r.ng + MF_NSP - 1) / MF_NSP;
+    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * per, i1 = min(i0 + per, r.ng);
     double s = 0.0;
-    for (int64_t i = threadIdx.x; i < r.ng; i += MF_THREADS) s += static_cast<double>(zr[wrap_index(i, r.n)]);
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += MF_THREADS) s += static_cast<double>(zr[wrap_index(i, r.n)]);
     s = block_sum(s, red);
-    if (threadIdx.x == 0) S[row] = s;
+    if (threadIdx.x == 0) Sp[row * MF_NSP + blockIdx.x] = s;
 }
 
 __global__ __launch_bounds__(MF_THREADS) void k_boxcar_part(
@@ -292,7 +312,8 @@ __global__ __launch_bounds__(MF_THREADS) void k_boxcar_part(
         return;
     }
     const float* __restrict__ zr = z + row * ldz;
-    const double srow = S[row];
+    double srow = 0.0;
+    for (int i = 0; i < MF_NSP; ++i) srow += S[row * MF_NSP + i];
     const double ng = static_cast<double>(r.ng);
     if (tid < nw) {
         const double w = widths[tid];
@@ -302,20 +323,42 @@ __global__ __launch_bounds__(MF_THREADS) void k_boxcar_part(
     }
     const int64_t lw = min<int64_t>(MF_B, r.n - a);
     const int nel = static_cast<int>(lw) + wmax;  // zt[a .. a + nel)
-    constexpr int KE = (MF_B + MF_WMAX + MF_THREADS - 1) / MF_THREADS;
+    // Each thread scans KE consecutive values (KE odd: its LDS reads and writes hit distinct
+    // banks). The values are first staged in Q's storage as floats with coalesced loads.
+    constexpr int KE = 13;
+    static_assert(KE * MF_THREADS >= MF_B + MF_WMAX && KE % 2 == 1, "odd run covering a block");
+    static_assert(KE * MF_THREADS * sizeof(float) <= sizeof(Q), "staging fits in Q");
+    float* Zs = reinterpret_cast<float*>(Q);
+    float zv[KE];
+    if (a + nel <= r.n) {
+        // interior block: all KE loads in flight at once
+#pragma unroll
+        for (int e = 0; e < KE; ++e) {
+            const int k = tid + MF_THREADS * e;
+            zv[e] = (k < nel) ? zr[a + k] : 0.f;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < KE; ++e) {
+            const int k = tid + MF_THREADS * e;
+            zv[e] = 0.f;
+            if (k < nel) {
+                const int64_t i = wrap_index(a + k, r.ng);  // a + k < n + MF_B + MF_WMAX
+                zv[e] = zr[wrap_index(i, r.n)];             // ng < 2n
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < KE; ++e) Zs[tid + MF_THREADS * e] = zv[e];
+    __syncthreads();
     double loc[KE];
     double tot = 0.0;
 #pragma unroll
     for (int e = 0; e < KE; ++e) {
-        const int k = tid * KE + e;
-        double v = 0.0;
-        if (k < nel) {
-            const int64_t i = wrap_index(a + k, r.ng);   // a + k < n + MF_B + MF_WMAX
-            v = static_cast<double>(zr[wrap_index(i, r.n)]);  // ng < 2n
-        }
-        tot += v;
+        tot += static_cast<double>(Zs[tid * KE + e]);
         loc[e] = tot;
     }
+    __syncthreads();  // Q is written below over the staged floats
     double ex = tot, dummy = 0.0;
     block_exscan2(ex, dummy, sa, sb);
     if (tid == 0) Q[0] = 0.0;
@@ -325,48 +368,36 @@ __global__ __launch_bounds__(MF_THREADS) void k_boxcar_part(
         if (k < nel) Q[k + 1] = ex + loc[e];
     }
     __syncthreads();
-    if (nw <= MF_NWREG) {
-        // Per-width maxima in registers: t increases within a thread, so a strict > keeps
-        // each width's first maximum; the template-major tie order is applied once below.
-        float bv[MF_NWREG];
-        int32_t bt[MF_NWREG];
+    // Widths outer (their constants are uniform), the thread's outputs u = tid + 256i inner,
+    // with Q[u] held in registers. t increases within a thread, so a strict > keeps each
+    // width's first maximum; better() then applies the template-major tie order.
+    constexpr int NU = MF_B / MF_THREADS;
+    double q0[NU];
 #pragma unroll
-        for (int wi = 0; wi < MF_NWREG; ++wi) {
-            bv[wi] = -FLT_MAX;
-            bt[wi] = -1;
-        }
-        for (int u = tid; u < lw; u += MF_THREADS) {
-            const double q0 = Q[u];
+    for (int i = 0; i < NU; ++i) {
+        const int u = tid + MF_THREADS * i;
+        q0[i] = (u < lw) ? Q[u] : 0.0;
+    }
+    for (int wi = 0; wi < nw; ++wi) {
+        const int w = sw[wi];
+        const double cm = coef_mean[wi], ci = coef_inv[wi];
+        float bv = -FLT_MAX;
+        int bt = -1;
 #pragma unroll
-            for (int wi = 0; wi < MF_NWREG; ++wi) {
-                if (wi < nw) {
-                    const float cv = static_cast<float>((Q[u + sw[wi]] - q0 - coef_mean[wi]) * coef_inv[wi]);
-                    if (cv > bv[wi]) {
-                        bv[wi] = cv;
-                        bt[wi] = u;
-                    }
+        for (int i = 0; i < NU; ++i) {
+            const int u = tid + MF_THREADS * i;
+            if (u < lw) {
+                const float cv = static_cast<float>((Q[u + w] - q0[i] - cm) * ci);
+                if (cv > bv) {
+                    bv = cv;
+                    bt = u;
                 }
             }
         }
-#pragma unroll
-        for (int wi = 0; wi < MF_NWREG; ++wi) {
-            if (wi < nw && bt[wi] >= 0 && better(bv[wi], wi, a + bt[wi], best)) {
-                best.val = bv[wi];
-                best.w = wi;
-                best.t = a + bt[wi];
-            }
-        }
-    } else {
-        for (int u = tid; u < lw; u += MF_THREADS) {
-            for (int wi = 0; wi < nw; ++wi) {
-                const double conv = (Q[u + sw[wi]] - Q[u] - coef_mean[wi]) * coef_inv[wi];
-                const float cv = static_cast<float>(conv);
-                if (better(cv, wi, a + u, best)) {
-                    best.val = cv;
-                    best.w = wi;
-                    best.t = a + u;
-                }
-            }
+        if (bt >= 0 && better(bv, wi, a + bt, best)) {
+            best.val = bv;
+            best.w = wi;
+            best.t = a + bt;
         }
     }
     bred[tid] = best;
@@ -458,19 +489,13 @@ int64_t good_size_real(int64_t n) {
 }
 
 struct NormLayout {
-    int64_t seg_pitch = 0, off_rows = 0, off_tot = 0, off_pre = 0, bytes = 0;
+    int64_t off_rows = 0, bytes = 0;
 };
 
-NormLayout norm_layout(int64_t nrows, const int64_t* row_len, int32_t window) {
+NormLayout norm_layout(int64_t nrows) {
     NormLayout l;
-    int64_t maxseg = 1;
-    for (int64_t i = 0; i < nrows; ++i)
-        maxseg = std::max<int64_t>(maxseg, (row_len[i] + window - 1 + RN_SEG - 1) / RN_SEG);
-    l.seg_pitch = maxseg;
     int64_t off = 0;
     l.off_rows = off; off = align256(off + static_cast<int64_t>(sizeof(RowDesc)) * nrows);
-    l.off_tot = off; off = align256(off + 16 * nrows * maxseg);
-    l.off_pre = off; off = align256(off + 16 * nrows * (maxseg + 1));
     l.bytes = off;
     return l;
 }
@@ -482,7 +507,7 @@ extern "C" int spp_running_normalise_workspace_size(int64_t nrows, const int64_t
     spp::clear_error();
     SPP_REQUIRE(nrows > 0 && row_len_host != nullptr && bytes != nullptr, "invalid argument");
     SPP_REQUIRE(window >= 1, "window must be >= 1");
-    *bytes = norm_layout(nrows, row_len_host, window).bytes;
+    *bytes = norm_layout(nrows).bytes;
     return SPP_OK;
 }
 
@@ -497,36 +522,26 @@ extern "C" int spp_running_normalise(const float* x, int64_t ldx, int64_t nrows,
     SPP_REQUIRE(window >= 1, "window must be >= 1");
     SPP_REQUIRE(static_cast<const void*>(x) != static_cast<const void*>(z), "z must not alias x");
     std::vector<RowDesc> rows(nrows);
-    int64_t maxseg = 0;
+    int64_t maxn = 0;
     for (int64_t i = 0; i < nrows; ++i) {
         SPP_REQUIRE(row_len_host[i] >= 1 && row_len_host[i] <= ldx && row_len_host[i] <= ldz,
                     "row %lld: length %lld outside [1, ld]", static_cast<long long>(i),
                     static_cast<long long>(row_len_host[i]));
         rows[i].n = row_len_host[i];
-        rows[i].nseg = (row_len_host[i] + window - 1 + RN_SEG - 1) / RN_SEG;
-        maxseg = std::max(maxseg, rows[i].nseg);
+        maxn = std::max(maxn, rows[i].n);
     }
-    const NormLayout l = norm_layout(nrows, row_len_host, window);
+    const NormLayout l = norm_layout(nrows);
     SPP_REQUIRE(workspace_bytes >= l.bytes, "workspace too small: %lld < %lld",
                 static_cast<long long>(workspace_bytes), static_cast<long long>(l.bytes));
-    SPP_REQUIRE(nrows <= 65535, "nrows must be <= 65535");
+    const int64_t nb_row = spp::ceil_div(maxn, int64_t(RN_CHB));
+    const int64_t nblocks = nrows * nb_row;
+    SPP_REQUIRE(nblocks + 8 <= 0x7fffffffLL, "too many rows x samples for one launch");
     hipStream_t st = spp::as_stream(stream);
     char* ws = static_cast<char*>(workspace);
     RowDesc* drows = reinterpret_cast<RowDesc*>(ws + l.off_rows);
-    double* tot = reinterpret_cast<double*>(ws + l.off_tot);
-    double* pre = reinterpret_cast<double*>(ws + l.off_pre);
     SPP_HIP(hipMemcpyAsync(drows, rows.data(), sizeof(RowDesc) * nrows, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_seg_totals, dim3(static_cast<unsigned>(maxseg), static_cast<unsigned>(nrows)),
-                       dim3(RN_THREADS), 0, st, x, ldx, drows, window, l.seg_pitch, tot);
-    SPP_LAUNCHED("k_seg_totals");
-    hipLaunchKernelGGL(k_seg_scan, dim3(static_cast<unsigned>(spp::ceil_div(nrows, 64))), dim3(64), 0, st,
-                       drows, nrows, l.seg_pitch, tot, pre);
-    SPP_LAUNCHED("k_seg_scan");
-    int64_t maxn = 0;
-    for (auto& r : rows) maxn = std::max(maxn, r.n);
-    hipLaunchKernelGGL(k_norm,
-                       dim3(static_cast<unsigned>(spp::ceil_div(maxn, RN_SEG)), static_cast<unsigned>(nrows)),
-                       dim3(RN_THREADS), 0, st, x, ldx, drows, window, l.seg_pitch, pre, z, ldz);
+    hipLaunchKernelGGL(k_norm, dim3(static_cast<unsigned>(spp::ceil_div(nblocks, int64_t(8)) * 8)),
+                       dim3(RN_THREADS), 0, st, x, ldx, drows, nb_row, nblocks, window, z, ldz);
     SPP_LAUNCHED("k_norm");
     // The row descriptors are read by the kernels above: keep the host copy alive.
     SPP_HIP(hipStreamSynchronize(st));
@@ -542,7 +557,7 @@ MfLayout mf_layout(int64_t nrows, int64_t max_len) {
     l.chunk_pitch = std::max<int64_t>(1, (max_len + MF_B - 1) / MF_B);
     int64_t off = 0;
     l.off_rows = off; off = align256(off + static_cast<int64_t>(sizeof(MfRow)) * nrows);
-    l.off_S = off; off = align256(off + 8 * nrows);
+    l.off_S = off; off = align256(off + 8 * MF_NSP * nrows);
     l.off_w = off; off = align256(off + 4 * MF_MAXW);
     l.off_part = off; off = align256(off + static_cast<int64_t>(sizeof(Best)) * nrows * l.chunk_pitch);
     l.bytes = off;
@@ -595,8 +610,8 @@ extern "C" int spp_boxcar_mf(const float* z, int64_t ldz, int64_t nrows, const i
     Best* part = reinterpret_cast<Best*>(ws + l.off_part);
     SPP_HIP(hipMemcpyAsync(drows, rows.data(), sizeof(MfRow) * nrows, hipMemcpyHostToDevice, st));
     SPP_HIP(hipMemcpyAsync(dw, widths_host, 4 * nwidths, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_circ_sum, dim3(static_cast<unsigned>(nrows)), dim3(MF_THREADS), 0, st, z, ldz,
-                       drows, S);
+    hipLaunchKernelGGL(k_circ_sum, dim3(MF_NSP, static_cast<unsigned>(nrows)), dim3(MF_THREADS), 0, st, z,
+                       ldz, drows, S);
     SPP_LAUNCHED("k_circ_sum");
     hipLaunchKernelGGL(k_boxcar_part,
                        dim3(static_cast<unsigned>(l.chunk_pitch), static_cast<unsigned>(nrows)),
