@@ -3,11 +3,11 @@
 //
 // Gulp layout: file order, [nsamps][nchans] packed nbits (1/2/4/8, bit order `big`)
 // or float32. A block owns DD_T consecutive output samples and a group of channels.
-// For each stage it picks the longest run of channels (at most DD_CC) whose delay
-// spread fits the LDS rows, stages rows [t0 + dmin, t0 + DD_T + dmax) of those
-// channels as one byte per sample (row stride 68 B = 17 dwords, so the per-sample
-// column reads below are bank-conflict free), and every thread adds its two outputs'
-// samples. Integer sums are exact (every partial sum < 2^24), so splitting channels
+// For each stage it picks the longest run of channels (at most 16 or DD_CC, the tile
+// width) whose delay spread fits the LDS rows, stages rows [t0 + dmin, t0 + DD_T + dmax)
+// of those channels as one byte per sample (rows of an odd number of dwords, so the
+// per-sample column reads below are bank-conflict free), and every thread adds its two
+// outputs' samples. Integer sums are exact (every partial sum < 2^24), so splitting channels
 // across blocks and combining with float atomics is bit-exact and order-free.
 //
 // Work per output sample: nchans adds; HBM bytes per gulp: nsamps * nchans * nbits/8
@@ -23,25 +23,10 @@ constexpr int DD_THREADS = 256;
 constexpr int DD_T = 512;       // output samples per block (2 per thread)
 constexpr int DD_CC = 64;       // channels per stage at most
 constexpr int DD_ROWS = 768;    // staged rows per stage: spread <= DD_ROWS - DD_T
-constexpr int DD_STRIDE = 68;   // bytes per staged row
-constexpr int DD_LD = 16;       // staged rows in flight per wave
+constexpr int DD_LD = 16;       // staging loads in flight per thread
 constexpr int DDF_CC = 16;      // float32 path: channels per stage
 constexpr int DDF_ROWS = 768;
 constexpr int DDF_STRIDE = 17;  // floats per staged row
-
-template <int NBITS, bool BIG>
-__device__ __forceinline__ uint32_t packed_sample(const uint8_t* __restrict__ in, int64_t row,
-                                                  int64_t row_bytes, int64_t c) {
-    if constexpr (NBITS == 8) {
-        return in[row * row_bytes + c];
-    } else {
-        constexpr int PER = 8 / NBITS;
-        const uint32_t b = in[row * row_bytes + c / PER];
-        const int k = static_cast<int>(c % PER);
-        const int shift = BIG ? (8 - (k + 1) * NBITS) : k * NBITS;
-        return (b >> shift) & ((1u << NBITS) - 1u);
-    }
-}
 
 // Greedy stage: the longest prefix of sdel[0..ncand) whose spread <= max_spread.
 __device__ __forceinline__ int stage_extent(const int32_t* sdel, int ncand, int max_spread,
@@ -61,18 +46,25 @@ __device__ __forceinline__ int stage_extent(const int32_t* sdel, int ncand, int 
 }
 
 // MODE 0: dedisperse (out[out_start + t] = sum). MODE 1: subband (out[t*nsubs + sub] += run sum).
-template <int NBITS, bool BIG, int MODE>
+// Staging: the stage's channels occupy bytes [bb0, bb1) of each packed row. All threads take
+// (row, unit) elements, a unit being a dword (WIDE: rows and the stream start are 4-byte
+// aligned) or a byte, DD_LD of them in flight per thread, and unpack every sample of a unit
+// that belongs to the stage into its tile column.
+template <int NBITS, bool BIG, int MODE, bool WIDE, int CCX>
 __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
     const uint8_t* __restrict__ in, const int32_t* __restrict__ delays,
     const int32_t* __restrict__ chan_to_sub, float* __restrict__ out, int64_t nchans, int64_t nsamps,
     int64_t nout, int64_t out_start, int64_t row_bytes, int64_t chans_per_group, int use_atomic,
     int64_t nsubs, int64_t out_size) {
-    __shared__ uint8_t tile[DD_ROWS * DD_STRIDE];
-    __shared__ int32_t sdel[DD_CC];
-    __shared__ int32_t ssub[DD_CC];
+    // CCX channels per stage at most; rows of CCX + 4 bytes (an odd number of dwords, so
+    // the per-sample column reads are bank-conflict free). Narrow channel groups use the
+    // small tile and fit more blocks per CU.
+    constexpr int STRIDE = CCX + 4;
+    static_assert((STRIDE / 4) % 2 == 1 && STRIDE % 4 == 0, "odd dword row stride");
+    __shared__ uint8_t tile[DD_ROWS * STRIDE];
+    __shared__ int32_t sdel[CCX];
+    __shared__ int32_t ssub[CCX];
     const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
     const int64_t t0 = static_cast<int64_t>(blockIdx.x) * DD_T;
     const int64_t cb = static_cast<int64_t>(blockIdx.y) * chans_per_group;
     const int64_t ce = min(nchans, cb + chans_per_group);
@@ -92,7 +84,7 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
     };
 
     for (int64_t c = cb; c < ce;) {
-        const int ncand = static_cast<int>(min<int64_t>(DD_CC, ce - c));
+        const int ncand = static_cast<int>(min<int64_t>(CCX, ce - c));
         __syncthreads();
         if (tid < ncand) {
             sdel[tid] = delays[c + tid];
@@ -103,24 +95,45 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
         const int n = stage_extent(sdel, ncand, DD_ROWS - DD_T, dmin, dmax);
         const int nrows = DD_T + (dmax - dmin);
         const int64_t rbase = t0 + dmin;
-        if (lane < n) {
-            // DD_LD rows per wave in flight: the loads are independent, the stores follow.
-            constexpr int WV = DD_THREADS / 64;
-            for (int r0 = wave; r0 < nrows; r0 += DD_LD * WV) {
+        {
+            constexpr int PER = 8 / NBITS;  // samples per byte
+            constexpr int UB = WIDE ? 4 : 1;  // bytes per unit
+            const int64_t bb0 = c * NBITS / 8, bb1 = ((c + n) * NBITS + 7) / 8;
+            const int64_t ub0 = bb0 / UB * UB;  // first unit's byte
+            const int nunit = static_cast<int>((bb1 - ub0 + UB - 1) / UB);
+            const int nel = nrows * nunit;
+            for (int e0 = 0; e0 < nel; e0 += DD_LD * DD_THREADS) {
                 uint32_t v[DD_LD];
 #pragma unroll
                 for (int u = 0; u < DD_LD; ++u) {
-                    const int r = r0 + u * WV;
+                    const int e = e0 + u * DD_THREADS + tid;
+                    const int r = e / nunit, q = e - r * nunit;
                     int64_t row = rbase + r;
                     if (row < 0) row += nsamps;  // numba wraparound of a negative index
                     v[u] = 0;
-                    if (r < nrows && row >= 0 && row < nsamps)
-                        v[u] = packed_sample<NBITS, BIG>(in, row, row_bytes, c + lane);
+                    if (e < nel && row >= 0 && row < nsamps) {
+                        const uint8_t* src = in + row * row_bytes + ub0 + UB * q;
+                        v[u] = WIDE ? *reinterpret_cast<const uint32_t*>(src) : *src;
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < DD_LD; ++u) {
-                    const int r = r0 + u * WV;
-                    if (r < nrows) tile[r * DD_STRIDE + lane] = static_cast<uint8_t>(v[u]);
+                    const int e = e0 + u * DD_THREADS + tid;
+                    if (e >= nel) break;
+                    const int r = e / nunit, q = e - r * nunit;
+#pragma unroll
+                    for (int jb = 0; jb < UB; ++jb) {
+                        const uint32_t byte = (v[u] >> (8 * jb)) & 0xffu;
+                        const int64_t ch0 = (ub0 + UB * q + jb) * PER;  // first channel of the byte
+#pragma unroll
+                        for (int k = 0; k < PER; ++k) {
+                            const int64_t ch = ch0 + k - c;
+                            if (ch >= 0 && ch < n) {
+                                const int shift = (NBITS == 8) ? 0 : (BIG ? (8 - (k + 1) * NBITS) : k * NBITS);
+                                tile[r * STRIDE + ch] = static_cast<uint8_t>((byte >> shift) & ((1u << NBITS) - 1u));
+                            }
+                        }
+                    }
                 }
             }
         }
@@ -134,8 +147,8 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
                 }
             }
             const int rel = sdel[k] - dmin;
-            acc0 += tile[(tid + rel) * DD_STRIDE + k];
-            acc1 += tile[(tid + DD_THREADS + rel) * DD_STRIDE + k];
+            acc0 += tile[(tid + rel) * STRIDE + k];
+            acc1 += tile[(tid + DD_THREADS + rel) * STRIDE + k];
         }
         c += n;
     }
@@ -267,10 +280,20 @@ int launch_int(const uint8_t* in, int nbits, int big, const int32_t* delays, con
         if (rc) return rc;
     }
     dim3 grid(static_cast<unsigned>(bx), static_cast<unsigned>(groups));
-#define SPP_GULP_LAUNCH(NB, BG)                                                               \
-    hipLaunchKernelGGL((k_gulp_int<NB, BG, MODE>), grid, dim3(DD_THREADS), 0, st, in, delays, cts, \
-                       out, nchans, nsamps, nout, out_start, row_bytes, cpg, use_atomic, nsubs,   \
-                       out_size)
+    const bool wide = row_bytes % 4 == 0 && reinterpret_cast<uintptr_t>(in) % 4 == 0;
+    const bool narrow = cpg <= 16;
+#define SPP_GULP_ONE(NB, BG, W, CX)                                                                 \
+    hipLaunchKernelGGL((k_gulp_int<NB, BG, MODE, W, CX>), grid, dim3(DD_THREADS), 0, st, in, delays, cts, \
+                       out, nchans, nsamps, nout, out_start, row_bytes, cpg, use_atomic, nsubs, out_size)
+#define SPP_GULP_LAUNCH(NB, BG)                  \
+    if (wide && narrow)                          \
+        SPP_GULP_ONE(NB, BG, true, 16);          \
+    else if (wide)                               \
+        SPP_GULP_ONE(NB, BG, true, DD_CC);       \
+    else if (narrow)                             \
+        SPP_GULP_ONE(NB, BG, false, 16);         \
+    else                                         \
+        SPP_GULP_ONE(NB, BG, false, DD_CC)
     switch (nbits * 2 + (big ? 1 : 0)) {
         case 2: SPP_GULP_LAUNCH(1, false); break;
         case 3: SPP_GULP_LAUNCH(1, true); break;
@@ -281,6 +304,7 @@ int launch_int(const uint8_t* in, int nbits, int big, const int32_t* delays, con
         default: SPP_GULP_LAUNCH(8, false); break;
     }
 #undef SPP_GULP_LAUNCH
+#undef SPP_GULP_ONE
     SPP_LAUNCHED("k_gulp_int");
     return SPP_OK;
 }

@@ -648,3 +648,51 @@ def test_serve_gulps_continue_rows(nbits, rng):
     assert start == total
     del st
     _lib.torch().cuda.synchronize()
+
+
+def test_gulp_loop_graph_replays_match_eager(rng):
+    """The one-DM gulp loop over resident data captured as a HIP graph: three replays, with
+    eager work and synchronisation between them, each equal bit for bit to the eager pass
+    (and to the oracle)."""
+    import torch
+
+    from sigpyproc3_amd import _lib
+    from sigpyproc3_amd.readers import plan_blocks
+
+    nchans, nsamps, nbits = 256, 40000, 2
+    samples = rng.integers(0, 4, nsamps * nchans, dtype=np.uint8)
+    packed = O.pack(samples, nbits, "big")
+    f = O.chan_freqs(nchans, -0.5, 1500.0)
+    delays = O.compute_dmdelays(f, 120.0, 64e-6, 1500.0).astype(np.int32)
+    md = int(delays.max())
+    gulp = max(2 * md, 8192)
+    _, blocks = plan_blocks(nsamps, gulp, md)
+    dev = _lib.to_device(packed)
+    d_dev = _lib.to_device(delays)
+    tim = _lib.zeros((nsamps - md,), "float32")
+    row_bytes = nchans * nbits // 8
+    calls, pos = [], 0
+    for ii, n, skip in blocks:
+        calls.append((ii, n, pos))
+        pos += n + skip
+
+    def one_pass(st):
+        for ii, n, p0 in calls:
+            _lib.call("spp_dedisperse", _lib.ptr(dev[p0 * row_bytes:]), nbits, 1, nchans, n,
+                      _lib.ptr(d_dev), md, _lib.ptr(tim), ii * (gulp - md), st)
+
+    one_pass(_lib.stream_handle())
+    torch.cuda.synchronize()
+    eager = tim.cpu().numpy()
+    ref = np.zeros(nsamps - md, np.float32)
+    c_oracle.dedisperse(samples, ref, delays, nchans, nsamps, 0)
+    np.testing.assert_array_equal(eager, ref)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        one_pass(_lib.stream_handle())
+    for rep in range(3):
+        tim.fill_(float(rep + 7))          # eager work between replays
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(tim.cpu().numpy(), eager, err_msg=f"replay {rep}")
