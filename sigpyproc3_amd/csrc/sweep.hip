@@ -24,7 +24,8 @@
 //                  and the raw words summed (even = raw - 16 * odd, recovered when byte
 //                  lanes are widened to 16-bit lanes every SB stages);
 //   byte fields (4-bit): summed over the stage, then widened to 16-bit lanes;
-//   8-bit data: even/odd bytes (v_and / v_perm) go straight to 16-bit lanes;
+//   8-bit data: odd bytes (v_perm) go straight to 16-bit lanes and the raw words are summed
+//                  as 32-bit integers; even = raw - 256 * odd when the lanes are read;
 //   32-bit lanes only when nchans * maxval > 65535.
 // Integer sums are exact, so the float32 output is bit-exact to the reference's float64 sums.
 //
@@ -272,8 +273,21 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
     };
     // Sample index (0..IL-1) of 16-bit lane l of accW[.][.][p][h].
     auto field_of = [](int p, int h, int l) { return (IL == 8) ? (4 * l + 2 * h + p) : (2 * l + h); };
+    // 8-bit data: accW[.][.][0][1] holds the odd bytes (16-bit lanes); accW[.][.][0][0] the plain
+    // 32-bit sum of the words, which wraps. A word is even + 256 * odd as integers, and both
+    // 16-bit-lane vectors stay below 2^32 while their lanes are below 2^16, so
+    // even = raw - (odd << 8) exactly modulo 2^32. evenW() turns the raw sums into the even lanes.
+    auto evenW = [&]() {
+        if constexpr (NBITS == 8) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) accW[k][q][0][0] -= accW[k][q][0][1] << 8;
+        }
+    };
     auto flushW = [&]() {
         if constexpr (USE32) {
+            evenW();
 #pragma unroll
             for (int k = 0; k < K; ++k)
 #pragma unroll
@@ -440,8 +454,8 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
                         if constexpr (NBITS == 4) {
                             a[q] += w[0][q] + w[1][q] + w[2][q] + w[3][q];  // byte lanes, <= CC * 15
                         } else {
-                            accW[k][q][0][0] += (w[0][q] & 0x00ff00ffu) + (w[1][q] & 0x00ff00ffu);
-                            accW[k][q][0][0] += (w[2][q] & 0x00ff00ffu) + (w[3][q] & 0x00ff00ffu);
+                            accW[k][q][0][0] += w[0][q] + w[1][q];  // raw sum (evenW)
+                            accW[k][q][0][0] += w[2][q] + w[3][q];
                             accW[k][q][0][1] += odd_bytes(w[0][q]) + odd_bytes(w[1][q]);
                             accW[k][q][0][1] += odd_bytes(w[2][q]) + odd_bytes(w[3][q]);
                         }
@@ -478,6 +492,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
         __syncthreads();
     }
     flushB();
+    evenW();
 
     // Store: field f of word (lane + 64q) is sample t0 + lane + 64q + f*S of DM row dm0 + wave*K + k.
 #pragma unroll
