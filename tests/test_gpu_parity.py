@@ -696,3 +696,64 @@ def test_gulp_loop_graph_replays_match_eager(rng):
         graph.replay()
         torch.cuda.synchronize()
         np.testing.assert_array_equal(tim.cpu().numpy(), eager, err_msg=f"replay {rep}")
+
+
+@pytest.mark.parametrize("nbits", [1, 2, 4, 8])
+@pytest.mark.parametrize("nchans", [256, 336])
+@pytest.mark.parametrize(("nsamps", "misalign"), [(2600, 0), (2600, 1), (60000, 0)])
+def test_packed_dedisperse_staging_paths(nbits, nchans, nsamps, misalign, rng):
+    """spp_dedisperse on packed input: short streams split channels into narrow groups (the
+    16-channel tile), long ones use the 64-channel tile; 336-channel 1/2-bit rows (42/84 B)
+    and a stream starting one byte off a word boundary take the byte-unit staging."""
+    from sigpyproc3_amd import _lib
+
+    order = O.DEFAULT_BITORDER.get(nbits, "big")
+    samples = rng.integers(0, 1 << nbits, nsamps * nchans, dtype=np.uint8)
+    packed = samples if nbits == 8 else O.pack(samples, nbits, order)
+    buf = np.zeros(packed.size + 4, np.uint8)
+    buf[misalign:misalign + packed.size] = packed
+    f = O.chan_freqs(nchans, -1.0, 1500.0)
+    delays = O.compute_dmdelays(f, 40.0 if nsamps < 10000 else 150.0, 64e-6, 1500.0).astype(np.int32)
+    md = int(delays.max())
+    dev = _lib.to_device(buf)
+    d_dev = _lib.to_device(delays)
+    out = _lib.zeros((nsamps - md,), "float32")
+    _lib.call("spp_dedisperse", _lib.ptr(dev[misalign:]), nbits, 1 if order == "big" else 0, nchans, nsamps,
+              _lib.ptr(d_dev), md, _lib.ptr(out), 0, _lib.stream_handle())
+    ref = np.zeros(nsamps - md, np.float32)
+    c_oracle.dedisperse(samples, ref, delays, nchans, nsamps, 0)
+    np.testing.assert_array_equal(out.cpu().numpy(), ref)
+
+
+def test_running_normalise_long_float_rows_and_wide_windows(rng):
+    """Rows spanning several normalisation blocks with non-integer data (the sliding window
+    sums round), and windows longer than the row (the mirrored padding covers the window)."""
+    from sigpyproc3_amd.sweep import running_normalise
+
+    x = (rng.normal(size=(3, 100000)) * 3.0 + 50.0).astype(np.float32)
+    row_len = np.array([100000, 70001, 3000])
+    for w in (4096, 1001):
+        z = running_normalise(x, row_len, w)
+        for i in range(3):
+            n = row_len[i]
+            np.testing.assert_allclose(z[i, :n], O.running_normalise(x[i, :n], w), rtol=1e-6, atol=2e-6)
+    y = rng.integers(0, 300, (2, 1500)).astype(np.float32)
+    z = running_normalise(y, np.array([1500, 700]), 4096)
+    for i, n in enumerate((1500, 700)):
+        np.testing.assert_allclose(z[i, :n], O.running_normalise(y[i, :n], 4096), rtol=2e-7, atol=1e-6)
+
+
+def test_boxcar_mf_many_widths(rng):
+    from sigpyproc3_amd.sweep import boxcar_mf
+
+    z = rng.normal(size=(3, 9000)).astype(np.float32)
+    z[1, 4000:4100] += 1.5
+    row_len = np.array([9000, 8500, 5000])
+    widths = list(range(1, 21)) + [50, 100, 200, 400, 1000]
+    snr, peak, widx = boxcar_mf(z, row_len, widths)
+    for i in range(3):
+        n = row_len[i]
+        conv = O.boxcar_conv_closed(z[i, :n], widths).astype(np.float32)
+        it, pk, v = O.mf_argmax(conv)
+        assert (widx[i], peak[i]) == (it, pk)
+        assert abs(snr[i] - v) <= 1e-5 * abs(v) + 1e-5 * np.sqrt(widths[it])
