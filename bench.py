@@ -410,6 +410,181 @@ def bench_file(args) -> int:
     return 0
 
 
+def bench_extras(args) -> int:
+    """The gulp pre-steps, fold and block operations on resident data: a 2-bit, 4096-channel,
+    2^18-sample stream (256 MiB packed) and a float32 block of 4096 channels x 16384 samples.
+    Each kernel is timed with HIP events (median of 10 launches) and reported as HBM bytes
+    per launch / time against 8 TB/s; the numpy oracle of the same operation is timed on a
+    bounded slice on the host beside it (per byte of packed input)."""
+    import time
+
+    import torch
+
+    from oracle import spp_oracle as O
+    from sigpyproc3_amd import _lib
+    from sigpyproc3_amd.foldedcube import band_starts, fold_cells
+    from sigpyproc3_amd.synth import SynthConfig, generate_packed
+
+    _lib.load()
+    _lib.require_gpu()
+    st = _lib.stream_handle()
+    cfg = SynthConfig(nchans=4096, nsamps=1 << 18, nbits=2, fch1=1550.0, foff=-0.0732421875,
+                      tsamp=64e-6, seed=7)
+    n, nch = cfg.nsamps, cfg.nchans
+    row_bytes = nch * cfg.nbits // 8
+    packed = generate_packed(cfg, device=_lib.device())
+    nbytes = int(packed.numel())
+    rng = np.random.default_rng(11)
+
+    def timed(fn, reps=10):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        return float(np.median(ts))
+
+    # host slice for the oracle timings: the first 4096 samples, unpacked
+    ns = 4096
+    host_slice = O.unpack(packed[: ns * row_bytes].cpu().numpy(), 2, "big")
+    slice_bytes = ns * row_bytes
+
+    def cpu_rate(fn):
+        t0 = time.perf_counter()
+        fn()
+        return slice_bytes / (time.perf_counter() - t0)
+
+    lines = {}
+
+    def record(name, sec, hbm_bytes, cpu_bps, note):
+        lines[name] = {"ms": round(sec * 1e3, 4), "hbm_bytes": int(hbm_bytes),
+                       "hbm_gbs": round(hbm_bytes / sec / 1e9, 1), "hbm_frac": round(hbm_bytes / sec / 8e12, 4),
+                       "input_gbs": round(nbytes / sec / 1e9, 1),
+                       "cpu_oracle_gbs": round(cpu_bps / 1e9, 4), "note": note}
+        log(f"extras: {name}: {sec * 1e3:.3f} ms, {hbm_bytes / sec / 1e9:.0f} GB/s "
+            f"({hbm_bytes / sec / 8e12:.3f} of 8 TB/s); oracle {cpu_bps / 1e9:.3f} GB/s of input")
+
+    # collapse: extract_tim = dedisperse with a zero delay table
+    zero = _lib.zeros((nch,), "int32")
+    tim = _lib.zeros((n,), "float32")
+    t = timed(lambda: _lib.call("spp_dedisperse", _lib.ptr(packed), 2, 1, nch, n, _lib.ptr(zero), 0,
+                                _lib.ptr(tim), 0, st))
+    record("collapse", t, nbytes + 4 * n,
+           cpu_rate(lambda: O.collapse(host_slice, ns, nch)), "one gulp of 2^18 samples")
+    # bandpass, float32 time order (extract_bpass) and per-gulp float64 (extract_bpass_par)
+    bp = _lib.zeros((nch,), "float32")
+    for par in (0, 1):
+        t = timed(lambda: _lib.call("spp_bandpass", _lib.ptr(packed), 2, 1, nch, n, _lib.ptr(bp), par, st))
+        record("bandpass_parallel" if par else "bandpass", t, nbytes + 4 * nch,
+               cpu_rate(lambda: O.bandpass(host_slice, ns, nch, parallel=bool(par))),
+               "per-channel sums over the gulp")
+    # channel mask on the packed bytes (in place on a copy)
+    mask = rng.random(nch) < 0.1
+    keep = np.zeros(row_bytes, np.uint32)
+    for k in range(4):
+        keep |= np.where(mask[k::4], 0, 3 << (8 - 2 * (k + 1))).astype(np.uint32)
+    keep_d = _lib.to_device(keep.astype(np.uint8))
+    set_d = _lib.zeros((row_bytes,), "uint8")
+    work = packed.clone()
+    t = timed(lambda: _lib.call("spp_mask_bytes", _lib.ptr(work), nbytes, row_bytes, _lib.ptr(keep_d),
+                                _lib.ptr(set_d), st))
+    record("mask_channels", t, 2 * nbytes,
+           cpu_rate(lambda: O.mask_channels(host_slice, nch, mask, 0)), "read + write the packed bytes")
+    # zero-DM removal: unpacked uint8 out
+    bpass = rng.uniform(1e5, 2e5, nch).astype(np.float32)
+    w_d = _lib.to_device(bpass / bpass.sum())
+    b_d = _lib.to_device(bpass)
+    zout = _lib.empty((n * nch,), "uint8")
+    t = timed(lambda: _lib.call("spp_remove_zerodm", _lib.ptr(packed), 2, 1, nch, n, _lib.ptr(w_d),
+                                _lib.ptr(b_d), _lib.ptr(zout), st))
+    zs = 64  # the oracle restates the reference's per-sample loop: a short slice
+    t0 = time.perf_counter()
+    O.remove_zerodm(host_slice[: zs * nch], nch, bpass, np.uint8)
+    zerodm_cpu = zs * row_bytes / (time.perf_counter() - t0)
+    record("remove_zerodm", t, nbytes + n * nch, zerodm_cpu,
+           "packed in, one byte per sample out; the oracle timed on 64 samples")
+    del zout, work
+    # fold: period 0.5 s, 64 bins, 16 subints, 32 bands, DM 100; cell lists built on the host
+    delays = cfg.delays([100.0])[0].astype(np.int32)
+    md = int(delays.max())
+    nvalid = n - md
+    nbins, nints, nbands = 64, 16, 32
+    subint, phasebin = fold_cells(0, nvalid, n, cfg.tsamp, 0.5, 0.0, nbins, nints)
+    key = subint * nbins + phasebin
+    order = np.argsort(key, kind="stable").astype(np.int32)
+    keys, first = np.unique(key[order], return_index=True)
+    key_start = np.append(first, nvalid).astype(np.int32)
+    key_pos = ((keys // nbins) * nbins * nbands + keys % nbins).astype(np.int32)
+    bufs = [_lib.to_device(x) for x in (delays, order, key_start, key_pos, band_starts(nch, nbands))]
+    fold_d = _lib.zeros((nbins * nints * nbands,), "float32")
+    count_d = _lib.zeros((nbins * nints * nbands,), "int32")
+
+    def fold_once():
+        fold_d.zero_()
+        count_d.zero_()
+        _lib.call("spp_fold", _lib.ptr(packed), 2, 1, nch, n, _lib.ptr(bufs[0]), _lib.ptr(bufs[1]), nvalid,
+                  _lib.ptr(bufs[2]), _lib.ptr(bufs[3]), len(keys), _lib.ptr(bufs[4]), nbands, nbins,
+                  _lib.ptr(fold_d), _lib.ptr(count_d), st)
+
+    t = timed(fold_once)
+    fs = 512
+    fa, fc = np.zeros(nbins * nints * nbands, np.float32), np.zeros(nbins * nints * nbands, np.int32)
+    t0 = time.perf_counter()
+    O.fold_kernel(host_slice[: fs * nch], fa, fc, delays.astype(np.int32) * 0, 0, cfg.tsamp, 0.5, 0.0, n,
+                  fs, nch, nbins, nints, nbands, 0)
+    fold_cpu = fs * row_bytes / (time.perf_counter() - t0)
+    record("fold", t, nbytes + 4 * order.size + 8 * fold_d.numel(), fold_cpu,
+           "every valid sample once, summed per (subint, band, bin) cell in list order; "
+           "the oracle is the reference's pure-Python loop on 512 samples")
+    # FilterbankBlock: roll every channel (dedisperse) and a 64-DM DM-time transform
+    blk_cols = 16384
+    blk = torch.randint(0, 4, (nch, blk_cols), dtype=torch.int32, device=_lib.device()).float()
+    bd = np.atleast_2d(cfg.delays([40.0])).astype(np.int64)[0]
+    off_h = np.ascontiguousarray(np.mod(bd, blk_cols), dtype=np.int64)
+    off_d = _lib.to_device(off_h)
+    rolled = _lib.empty((nch, blk_cols), "float32")
+    t = timed(lambda: _lib.call("spp_roll_rows", _lib.ptr(blk), nch, blk_cols, _lib.ptr(off_h), _lib.ptr(off_d),
+                                blk_cols, _lib.ptr(rolled), st))
+    hb = blk[:, :1024].cpu().numpy()
+    t0 = time.perf_counter()
+    O.roll_block(hb, -bd)
+    roll_cpu = hb.nbytes / (time.perf_counter() - t0)
+    record("block_dedisperse", t, 2 * blk.numel() * 4, roll_cpu,
+           "float32 block 4096 x 16384 rolled per channel (block bytes in + out)")
+    ndm = 64
+    dmd = np.atleast_2d(cfg.delays(40.0 + np.linspace(-40.0, 40.0, ndm))).astype(np.int64)
+    offs_h = np.ascontiguousarray(np.mod(-dmd, blk_cols), dtype=np.int64)
+    offs_d = _lib.to_device(offs_h)
+    dmt = _lib.empty((ndm, blk_cols), "float32")
+    t = timed(lambda: _lib.call("spp_dmt", _lib.ptr(blk), nch, blk_cols, _lib.ptr(offs_h), _lib.ptr(offs_d), ndm,
+                                blk_cols, _lib.ptr(dmt), st), reps=5)
+    t0 = time.perf_counter()
+    O.dmt_block(hb, dmd[:4])
+    dmt_cpu = 4 * hb.nbytes / (time.perf_counter() - t0)
+    record("block_dmt_64dm", t, ndm * blk.numel() * 4 + dmt.numel() * 4, dmt_cpu,
+           "each of 64 DMs reads the whole block (float32 channel sums in channel order)")
+    result = {
+        "metric": "gulp pre-steps, fold and block operations: HBM GB/s per kernel",
+        "value": lines["remove_zerodm"]["hbm_gbs"],
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "u8/f32",
+        "data": "synthetic",
+        "config": {"workload": "2-bit 4096 ch x 2^18 samples (256 MiB packed) resident in HBM; float32 block "
+                               "4096 x 16384; value = remove_zerodm's HBM rate"},
+        "kernels": lines,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -423,11 +598,12 @@ def main() -> int:
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="first run a plain device copy of this many GiB three times (the "
                          "reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--workload", choices=("sweep", "serve", "mf", "file", "single"), default="sweep",
+    ap.add_argument("--workload", choices=("sweep", "serve", "mf", "file", "single", "extras"), default="sweep",
                     help="sweep: the 2-bit 2048-DM sweep (the headline line); serve: the "
                          "16-stream gulp service with p50/p99 latency; mf: the 8-bit 1024-DM "
                          "sweep + normalisation + boxcar matched filter; file: the same stream "
-                         "read from a SIGPROC file; single: the one-DM gulp path and unpack")
+                         "read from a SIGPROC file; single: the one-DM gulp path and unpack; "
+                         "extras: the gulp pre-steps, fold and block operations")
     ap.add_argument("--serve-gulps", type=int, default=160, help="gulps per serve run")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -443,6 +619,8 @@ def main() -> int:
         return bench_file(args)
     if args.workload == "single":
         return bench_single(args)
+    if args.workload == "extras":
+        return bench_extras(args)
 
     import torch
 
