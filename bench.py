@@ -469,20 +469,20 @@ def bench_extras(args) -> int:
         log(f"extras: {name}: {sec * 1e3:.3f} ms, {hbm_bytes / sec / 1e9:.0f} GB/s "
             f"({hbm_bytes / sec / 8e12:.3f} of 8 TB/s); oracle {cpu_bps / 1e9:.3f} GB/s of input")
 
-    # collapse: extract_tim = dedisperse with a zero delay table
-    zero = _lib.zeros((nch,), "int32")
+    # collapse (extract_tim): per-sample channel sums
     tim = _lib.zeros((n,), "float32")
-    t = timed(lambda: _lib.call("spp_dedisperse", _lib.ptr(packed), 2, 1, nch, n, _lib.ptr(zero), 0,
-                                _lib.ptr(tim), 0, st))
+    t = timed(lambda: _lib.call("spp_collapse", _lib.ptr(packed), 2, 1, nch, n, _lib.ptr(tim), 0, st))
     record("collapse", t, nbytes + 4 * n,
            cpu_rate(lambda: O.collapse(host_slice, ns, nch)), "one gulp of 2^18 samples")
     # bandpass, float32 time order (extract_bpass) and per-gulp float64 (extract_bpass_par)
     bp = _lib.zeros((nch,), "float32")
+    bp_ws = _lib.workspace(_lib.workspace_size("spp_bandpass_workspace_size", 2, nch, n))
     for par in (0, 1):
-        t = timed(lambda: _lib.call("spp_bandpass", _lib.ptr(packed), 2, 1, nch, n, _lib.ptr(bp), par, st))
-        record("bandpass_parallel" if par else "bandpass", t, nbytes + 4 * nch,
+        t = timed(lambda: _lib.call("spp_bandpass", _lib.ptr(packed), 2, 1, nch, n, _lib.ptr(bp), par,
+                                    _lib.ptr(bp_ws), bp_ws.numel(), st))
+        record("bandpass_parallel" if par else "bandpass", t, nbytes + 4 * nch + 2 * bp_ws.numel(),
                cpu_rate(lambda: O.bandpass(host_slice, ns, nch, parallel=bool(par))),
-               "per-channel sums over the gulp")
+               "per-channel sums over the gulp (HBM bytes include the chunk partials written and read)")
     # channel mask on the packed bytes (in place on a copy)
     mask = rng.random(nch) < 0.1
     keep = np.zeros(row_bytes, np.uint32)
@@ -523,13 +523,14 @@ def bench_extras(args) -> int:
     bufs = [_lib.to_device(x) for x in (delays, order, key_start, key_pos, band_starts(nch, nbands))]
     fold_d = _lib.zeros((nbins * nints * nbands,), "float32")
     count_d = _lib.zeros((nbins * nints * nbands,), "int32")
+    fold_ws = _lib.workspace(_lib.workspace_size("spp_fold_workspace_size", 2, nch, n, nbands))
 
     def fold_once():
         fold_d.zero_()
         count_d.zero_()
         _lib.call("spp_fold", _lib.ptr(packed), 2, 1, nch, n, _lib.ptr(bufs[0]), _lib.ptr(bufs[1]), nvalid,
                   _lib.ptr(bufs[2]), _lib.ptr(bufs[3]), len(keys), _lib.ptr(bufs[4]), nbands, nbins,
-                  _lib.ptr(fold_d), _lib.ptr(count_d), st)
+                  _lib.ptr(fold_d), _lib.ptr(count_d), _lib.ptr(fold_ws), fold_ws.numel(), st)
 
     t = timed(fold_once)
     fs = 512
@@ -538,8 +539,8 @@ def bench_extras(args) -> int:
     O.fold_kernel(host_slice[: fs * nch], fa, fc, delays.astype(np.int32) * 0, 0, cfg.tsamp, 0.5, 0.0, n,
                   fs, nch, nbins, nints, nbands, 0)
     fold_cpu = fs * row_bytes / (time.perf_counter() - t0)
-    record("fold", t, nbytes + 4 * order.size + 8 * fold_d.numel(), fold_cpu,
-           "every valid sample once, summed per (subint, band, bin) cell in list order; "
+    record("fold", t, nbytes + 4 * order.size + 8 * fold_d.numel() + 8 * n * nbands, fold_cpu,
+           "band sums of every sample (fused gulp kernel), then per (subint, band, bin) cell totals; "
            "the oracle is the reference's pure-Python loop on 512 samples")
     # FilterbankBlock: roll every channel (dedisperse) and a 64-DM DM-time transform
     blk_cols = 16384

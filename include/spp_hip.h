@@ -136,11 +136,22 @@ int spp_sweep_execute_n(const spp_sweep_plan* plan, const uint8_t* packed, int64
  * spp_bandpass: per-channel sums of one gulp added into out[nchans] (device, float32).
  * parallel == 0 replaces kernels.extract_bpass (core/kernels.py:277-291): float32 adds in
  * time order per channel. parallel != 0 replaces kernels.extract_bpass_par
- * (core/kernels.py:294-310): out[c] += float32(float64 sum over the gulp). Called by
- * Filterbank.bandpass (base.py:308-358).
+ * (core/kernels.py:294-310): out[c] = float32(float64(out[c]) + float64 sum over the gulp),
+ * numba's float32 += float64. Called by Filterbank.bandpass (base.py:308-358). Integer data
+ * need a device workspace of spp_bandpass_workspace_size bytes (none for float32); a
+ * workspace sized for nsamps serves every shorter gulp.
  */
+int spp_bandpass_workspace_size(int nbits, int64_t nchans, int64_t nsamps, int64_t* bytes);
 int spp_bandpass(const void* in, int nbits, int big, int64_t nchans, int64_t nsamps, float* out,
-                 int parallel, void* stream);
+                 int parallel, void* workspace, int64_t workspace_bytes, void* stream);
+/*
+ * spp_collapse: out[out_start + t] = float32(sum over c of x[t][c]), the float64 channel sum
+ * of kernels.extract_tim (core/kernels.py:255-273) as called by Filterbank.collapse
+ * (base.py:256-306). Integer sums are exact (nchans < 2^24); float32 data are summed in
+ * float64 in channel order.
+ */
+int spp_collapse(const void* in, int nbits, int big, int64_t nchans, int64_t nsamps, float* out,
+                 int64_t out_start, void* stream);
 /*
  * spp_mask_bytes: in place, byte j of every row_bytes-long row becomes
  * (b & keep[j]) | set[j] (keep/set: device uint8 [row_bytes]). With keep/set built from a
@@ -168,13 +179,15 @@ int spp_remove_zerodm(const void* in, int nbits, int big, int64_t nchans, int64_
  * float32 sum (fold) and count over its samples in time order and the band's channels in
  * channel order, reading x[s + delays[c]][c] of the packed gulp (nbits 1/2/4/8 with bit order
  * `big`, or 32 for float32); a negative row wraps by nsamps. Host arrays: none; all pointers
- * are device pointers. fold/count: float32/int32 [nints * nbands * nbins].
+ * are device pointers. fold/count: float32/int32 [nints * nbands * nbins]. Integer data need a
+ * device workspace of spp_fold_workspace_size bytes (the gulp's band sums; none for float32).
  */
+int spp_fold_workspace_size(int nbits, int64_t nchans, int64_t nsamps, int32_t nbands, int64_t* bytes);
 int spp_fold(const void* in, int nbits, int big, int64_t nchans, int64_t nsamps,
              const int32_t* delays, const int32_t* samples, int64_t nlisted,
              const int32_t* key_start, const int32_t* key_pos, int32_t nkeys,
              const int32_t* band_start, int32_t nbands, int32_t nbins, float* fold, int32_t* count,
-             void* stream);
+             void* workspace, int64_t workspace_bytes, void* stream);
 
 /*
  * In-memory blocks: float32 [nrows][ncols] (channel-major FilterbankBlock data). Row r is

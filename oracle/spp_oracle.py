@@ -314,7 +314,8 @@ def bandpass(stream, nsamples: int, nchans: int, gulp=16384, start=0, nsamps=Non
 
     parallel=False: kernels.extract_bpass (kernels.py:277-291), ``out[c] += x[t][c]`` in
     float32 in time order. parallel=True: kernels.extract_bpass_par (kernels.py:294-310),
-    ``out[c] += float32(sum_t float64(x[t][c]))`` per gulp.
+    ``out_arr[ichan] += summ`` with a float64 ``summ``: numba adds float32 + float64 in
+    float64 and rounds once on the float32 store, per gulp.
     """
     n = (nsamples - start) if nsamps is None else nsamps
     out = np.zeros(nchans, dtype=np.float32)
@@ -324,7 +325,7 @@ def bandpass(stream, nsamples: int, nchans: int, gulp=16384, start=0, nsamps=Non
     for _ii, nsamps_r, skip in read_plan_blocks(n, gulp, 0):
         rows = x[pos : pos + nsamps_r]
         if parallel:
-            out += rows.astype(np.float64).sum(axis=0).astype(np.float32)
+            out = (out.astype(np.float64) + rows.astype(np.float64).sum(axis=0)).astype(np.float32)
         else:
             for t in range(nsamps_r):
                 out += rows[t].astype(np.float32)

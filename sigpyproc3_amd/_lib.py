@@ -57,10 +57,14 @@ SIGNATURES: dict[str, tuple[object, list]] = {
     "spp_sweep_build": (_int, [_p, _p, _p, _i64, _p]),
     "spp_sweep_run": (_int, [_p, _p, _i64, _p, _i64, _p]),
     "spp_sweep_execute_n": (_int, [_p, _p, _i64, _p, _i64, _p, _i64, _p]),
-    "spp_bandpass": (_int, [_p, _int, _int, _i64, _i64, _p, _int, _p]),
+    "spp_bandpass_workspace_size": (_int, [_int, _i64, _i64, _P64]),
+    "spp_bandpass": (_int, [_p, _int, _int, _i64, _i64, _p, _int, _p, _i64, _p]),
+    "spp_collapse": (_int, [_p, _int, _int, _i64, _i64, _p, _i64, _p]),
     "spp_mask_bytes": (_int, [_p, _i64, _i64, _p, _p, _p]),
     "spp_remove_zerodm": (_int, [_p, _int, _int, _i64, _i64, _p, _p, _p, _p]),
-    "spp_fold": (_int, [_p, _int, _int, _i64, _i64, _p, _p, _i64, _p, _p, _i32, _p, _i32, _i32, _p, _p, _p]),
+    "spp_fold_workspace_size": (_int, [_int, _i64, _i64, _i32, _P64]),
+    "spp_fold": (_int, [_p, _int, _int, _i64, _i64, _p, _p, _i64, _p, _p, _i32, _p, _i32, _i32, _p, _p, _p, _i64,
+                        _p]),
     "spp_roll_rows": (_int, [_p, _i64, _i64, _p, _p, _i64, _p, _p]),
     "spp_dmt": (_int, [_p, _i64, _i64, _p, _p, _i64, _i64, _p, _p]),
     "spp_running_normalise_workspace_size": (_int, [_i64, _p, _i32, _P64]),
@@ -109,6 +113,18 @@ def check(rc: int) -> None:
 
 def call(name: str, *args) -> None:
     check(getattr(load(), name)(*args))
+
+
+def workspace_size(name: str, *args) -> int:
+    """Bytes reported by a ``*_workspace_size`` entry point (its last argument is the output)."""
+    out = np.zeros(1, dtype=np.int64)
+    call(name, *args, out.ctypes.data_as(_P64))
+    return int(out[0])
+
+
+def workspace(nbytes: int):
+    """A device byte buffer of at least ``nbytes`` (one byte when none is needed)."""
+    return empty((max(int(nbytes), 1),), "uint8")
 
 
 # ----------------------------------------------------------------------------- torch plumbing

@@ -244,9 +244,9 @@ class Filterbank(ABC):
                  nsamps: int | None = None, *, parallel: bool = False, **plan_kwargs) -> TimeSeries:
         """Zero-DM time series: the sum over channels per sample (base.py:256-306).
 
-        ``kernels.extract_tim`` is ``dedisperse`` with every delay 0, so this runs the same
-        fused kernel (``spp_dedisperse``) with a zero delay table. Gulp ``ii`` is written at
-        offset ``ii * nsamps_r``, as the reference does (base.py:297-303).
+        Runs ``spp_collapse`` (``kernels.extract_tim``: the float64 channel sum of each sample,
+        exact for integer data) per gulp. Gulp ``ii`` is written at offset ``ii * nsamps_r``, as
+        the reference does (base.py:297-303).
         """
         del parallel
         hdr = self.header
@@ -254,14 +254,13 @@ class Filterbank(ABC):
         nbits, big = self._kernel_nbits()
         _lib.require_gpu()
         tim = _lib.zeros((max(tim_len, 1),), "float32")
-        zero = _lib.zeros((hdr.nchans,), "int32")
         stream = _lib.stream_handle()
         for nsamps_r, ii, dev, _nb in self._device_gulps(gulp, start, nsamps, plan_kwargs):
             if ii * nsamps_r + nsamps_r > tim_len:
                 msg = "output offset beyond the time series"
                 raise ValueError(msg)
-            _lib.call("spp_dedisperse", _lib.ptr(dev), nbits, big, hdr.nchans, nsamps_r,
-                      _lib.ptr(zero), 0, _lib.ptr(tim), ii * nsamps_r, stream)
+            _lib.call("spp_collapse", _lib.ptr(dev), nbits, big, hdr.nchans, nsamps_r, _lib.ptr(tim),
+                      ii * nsamps_r, stream)
         tim_ar = tim[:tim_len].cpu().numpy() if tim_len > 0 else np.zeros(0, dtype=np.float32)
         if method == "mean":
             tim_ar /= hdr.nchans
@@ -280,10 +279,14 @@ class Filterbank(ABC):
         _lib.require_gpu()
         bpass = _lib.zeros((hdr.nchans,), "float32")
         stream = _lib.stream_handle()
+        ws = None
         num_samples = 0
         for nsamps_r, _ii, dev, _nb in self._device_gulps(gulp, start, nsamps, plan_kwargs):
+            need = _lib.workspace_size("spp_bandpass_workspace_size", nbits, hdr.nchans, nsamps_r)
+            if ws is None or ws.numel() < need:
+                ws = _lib.workspace(need)
             _lib.call("spp_bandpass", _lib.ptr(dev), nbits, big, hdr.nchans, nsamps_r,
-                      _lib.ptr(bpass), 1 if parallel else 0, stream)
+                      _lib.ptr(bpass), 1 if parallel else 0, _lib.ptr(ws), ws.numel(), stream)
             num_samples += nsamps_r
         bpass_ar = bpass.cpu().numpy()
         if method == "mean":
@@ -393,7 +396,8 @@ class Filterbank(ABC):
 
         Per gulp the host lists the valid samples by (subint, phase bin) with the
         reference's float64 arithmetic (``foldedcube.fold_cells``); ``spp_fold`` then sums
-        every cell in the reference's order. Returns ``FoldedData``.
+        every cell to the reference's float32 value (exact integer totals while a cell stays
+        below 2^24, the reference's order beyond). Returns ``FoldedData``.
         """
         from sigpyproc3_amd.foldedcube import FoldedData, band_starts, fold_cells
 
@@ -418,6 +422,7 @@ class Filterbank(ABC):
         bands_dev = _lib.to_device(band_starts(hdr.nchans, nbands))
         stream = _lib.stream_handle()
         keep = []
+        ws = None
         for nsamps_r, ii, dev, _nb in self._device_gulps(gulp, start, nsamps, plan_kwargs, max_delay):
             nvalid = nsamps_r - max_delay
             if nvalid <= 0:
@@ -434,9 +439,13 @@ class Filterbank(ABC):
             key_pos = ((keys // nbins) * nbins * nbands + keys % nbins).astype(np.int32)
             bufs = [_lib.to_device(order), _lib.to_device(key_start), _lib.to_device(key_pos)]
             keep.append(bufs)
+            need = _lib.workspace_size("spp_fold_workspace_size", nbits, hdr.nchans, nsamps_r, nbands)
+            if ws is None or ws.numel() < need:
+                ws = _lib.workspace(need)
             _lib.call("spp_fold", _lib.ptr(dev), nbits, big, hdr.nchans, nsamps_r, _lib.ptr(d_dev),
                       _lib.ptr(bufs[0]), nvalid, _lib.ptr(bufs[1]), _lib.ptr(bufs[2]), len(keys),
-                      _lib.ptr(bands_dev), nbands, nbins, _lib.ptr(fold_d), _lib.ptr(count_d), stream)
+                      _lib.ptr(bands_dev), nbands, nbins, _lib.ptr(fold_d), _lib.ptr(count_d),
+                      _lib.ptr(ws), ws.numel(), stream)
         fold_ar = fold_d.cpu().numpy()
         count_ar = count_d.cpu().numpy()
         del keep

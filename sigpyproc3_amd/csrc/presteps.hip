@@ -5,7 +5,9 @@
 // float32 (nbits == 32). All three are HBM-bound single passes over the gulp.
 #include "spp_common.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace {
 
@@ -24,20 +26,267 @@ __device__ __forceinline__ float sample_at(const void* in, int64_t row, int64_t 
     }
 }
 
-// Bandpass of one gulp, one thread per channel, time in order.
-//   SEQ:  out[c] = (...((out[c] + x[0][c]) + x[1][c]) ...) in float32, the order of
-//         kernels.extract_bpass (core/kernels.py:277-291);
-//   !SEQ: out[c] += float32(sum_t x[t][c] in float64), kernels.extract_bpass_par
-//         (core/kernels.py:294-310).
+// ---------------------------------------------------------------------------------------
+// Integer gulps (nbits 1/2/4/8). Every sample is a small non-negative integer, so a float32
+// running sum that starts at an integer o with |o| <= 2^24 and ends at |o + T| <= 2^24 never
+// rounds: every partial sum is an integer of at most 24 bits. In that range any summation
+// order gives the reference's float32 result, and the sums below are exact integer
+// reductions in whatever order the hardware prefers.
+
+// Sum of every field of a packed word (bit order does not matter for a sum): 1-bit popcount,
+// 2-bit popcount of the word plus popcount of the high bits, 4/8-bit byte SAD against zero.
+template <int NBITS>
+__device__ __forceinline__ uint32_t field_sum(uint32_t w, uint32_t acc) {
+    if constexpr (NBITS == 1) {
+        return acc + static_cast<uint32_t>(__builtin_popcount(w));
+    } else if constexpr (NBITS == 2) {
+        return acc + static_cast<uint32_t>(__builtin_popcount(w)) +
+               static_cast<uint32_t>(__builtin_popcount(w & 0xaaaaaaaau));
+    } else if constexpr (NBITS == 4) {
+        return __builtin_amdgcn_sad_u8(w & 0x0f0f0f0fu, 0u,
+                                       __builtin_amdgcn_sad_u8((w >> 4) & 0x0f0f0f0fu, 0u, acc));
+    } else {
+        return __builtin_amdgcn_sad_u8(w, 0u, acc);
+    }
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o));
+    return v;
+}
+
+// Collapse (kernels.extract_tim, core/kernels.py:255-273) of an integer gulp: one wave per
+// CL_ROWS rows, UB bytes per lane load; out[t] = float32 of the exact row sum (the
+// reference's float64 sum of integers, rounded once on the store).
+constexpr int CL_ROWS = 4;
+template <int NBITS, int UB>
+__global__ __launch_bounds__(256) void k_collapse(const uint8_t* __restrict__ in, int64_t nsamps,
+                                                  int64_t row_bytes, float* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t r0 = (static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6)) * CL_ROWS;
+    const int64_t nunit = row_bytes / UB;
+    uint32_t s[CL_ROWS] = {};
+    for (int64_t u = lane; u < nunit; u += 64) {
+        uint32_t v[CL_ROWS][UB >= 4 ? UB / 4 : 1];
+#pragma unroll
+        for (int i = 0; i < CL_ROWS; ++i) {
+            const int64_t r = min(r0 + i, nsamps - 1);  // rows past the end re-read the last one
+            const uint8_t* p = in + r * row_bytes + u * UB;
+            if constexpr (UB == 16) {
+                const uint4 q = *reinterpret_cast<const uint4*>(p);
+                v[i][0] = q.x;
+                v[i][1] = q.y;
+                v[i][2] = q.z;
+                v[i][3] = q.w;
+            } else if constexpr (UB == 4) {
+                v[i][0] = *reinterpret_cast<const uint32_t*>(p);
+            } else {
+                v[i][0] = *p;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < CL_ROWS; ++i) {
+#pragma unroll
+            for (int k = 0; k < (UB >= 4 ? UB / 4 : 1); ++k) s[i] = field_sum<NBITS>(v[i][k], s[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < CL_ROWS; ++i) {
+        const uint32_t t = wave_sum(s[i]);
+        if (lane == 0 && r0 + i < nsamps) out[r0 + i] = static_cast<float>(t);
+    }
+}
+
+// Collapse of float32 rows: one thread per row, float64 sum in channel order.
+__global__ __launch_bounds__(256) void k_collapse_f32(const float* __restrict__ in, int64_t nsamps,
+                                                      int64_t nchans, float* __restrict__ out) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= nsamps) return;
+    const float* row = in + t * nchans;
+    double s = 0.0;
+    for (int64_t c = 0; c < nchans; ++c) s += static_cast<double>(row[c]);
+    out[t] = static_cast<float>(s);
+}
+
+// Bandpass, pass 1: per-channel integer sums of one chunk of rows. A block covers 64 lane
+// units of a row (VW dwords each, or one byte when VW == 0) and chunk_rows rows; its four
+// waves take every fourth row. Samples are added as SWAR fields (8-bit lanes for 1/2-bit
+// data, 16-bit lanes for 4/8-bit), widened into per-channel counters before a lane can
+// overflow, and the waves' counters are combined in LDS. part[chunk][c] is the chunk's sum.
+constexpr int BP_R = 8;          // rows in flight per lane
+constexpr int BP_BLOCKS = 1024;  // target pass-1 grid
+template <int NBITS, bool BIG, int VW>
+__global__ __launch_bounds__(256) void k_bp_part(const uint8_t* __restrict__ in, int64_t nsamps,
+                                                 int64_t row_bytes, int64_t chunk_rows, int64_t nchans,
+                                                 uint32_t* __restrict__ part) {
+    constexpr int PER = 8 / NBITS;
+    constexpr int NW = VW ? VW : 1;          // words per lane unit
+    constexpr int UB = VW ? 4 * VW : 1;      // bytes per lane unit
+    constexpr int NB = VW ? 4 : 1;           // bytes used per word
+    constexpr int CW = NB * PER;             // channels per word
+    constexpr int CPU = NW * CW;             // channels per lane unit
+    constexpr bool BL = NBITS <= 2;          // 8-bit lanes (else 16-bit lanes)
+    constexpr int NA = BL ? PER : 2 * PER;   // SWAR accumulators per word
+    constexpr int FL = BL ? 64 : 256;        // rows between widenings: 3*64 < 2^8, 255*256 < 2^16
+    constexpr uint32_t FM = NBITS == 8 ? 0xffu : (1u << NBITS) - 1u;
+    constexpr uint32_t LM = BL ? FM * 0x01010101u : FM * 0x00010001u;
+    constexpr int LSTR = CPU + 1;            // odd LDS stride per lane unit
+    __shared__ uint32_t red[64 * LSTR];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t nunit = row_bytes / UB;
+    const int64_t unit = static_cast<int64_t>(blockIdx.x) * 64 + lane;
+    const bool active = unit < nunit;
+    const int64_t r0 = static_cast<int64_t>(blockIdx.y) * chunk_rows;
+    const int64_t r1 = min(nsamps, r0 + chunk_rows);
+    uint32_t acc[NW][NA];
+    uint32_t tot[NW][CW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) acc[w][a] = 0;
+#pragma unroll
+        for (int j = 0; j < CW; ++j) tot[w][j] = 0;
+    }
+    auto widen = [&]() {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                const int k = BL ? a : (a >> 1);
+                const int ck = BIG ? (PER - 1 - k) : k;
+                if constexpr (BL) {
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) tot[w][b * PER + ck] += (acc[w][a] >> (8 * b)) & 0xffu;
+                } else {
+#pragma unroll
+                    for (int hf = 0; hf < 2; ++hf) {
+                        const int b = (a & 1) + 2 * hf;
+                        if (b < NB) tot[w][b * PER + ck] += (acc[w][a] >> (16 * hf)) & 0xffffu;
+                    }
+                }
+                acc[w][a] = 0;
+            }
+        }
+    };
+    int pend = 0;
+    for (int64_t r = r0 + wv; r < r1; r += 4 * BP_R) {
+        uint32_t x[BP_R][NW];
+#pragma unroll
+        for (int i = 0; i < BP_R; ++i) {
+            const int64_t row = r + 4 * i;
+            const bool ok = active && row < r1;
+            const uint8_t* p = in + (ok ? row * row_bytes + unit * UB : 0);
+            if constexpr (VW == 4) {
+                uint4 q = make_uint4(0, 0, 0, 0);
+                if (ok) q = *reinterpret_cast<const uint4*>(p);
+                x[i][0] = q.x;
+                x[i][1] = q.y;
+                x[i][2] = q.z;
+                x[i][3] = q.w;
+            } else if constexpr (VW == 2) {
+                uint2 q = make_uint2(0, 0);
+                if (ok) q = *reinterpret_cast<const uint2*>(p);
+                x[i][0] = q.x;
+                x[i][1] = q.y;
+            } else if constexpr (VW == 1) {
+                x[i][0] = ok ? *reinterpret_cast<const uint32_t*>(p) : 0u;
+            } else {
+                x[i][0] = ok ? static_cast<uint32_t>(*p) : 0u;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < BP_R; ++i) {
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                    const int sh = BL ? a * NBITS : (8 * (a & 1) + (a >> 1) * NBITS);
+                    acc[w][a] += (x[i][w] >> sh) & LM;
+                }
+            }
+        }
+        pend += BP_R;
+        if (pend + BP_R > FL) {
+            widen();
+            pend = 0;
+        }
+    }
+    widen();
+    if (wv == 0) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+#pragma unroll
+            for (int j = 0; j < CW; ++j) red[lane * LSTR + w * CW + j] = tot[w][j];
+        }
+    }
+    __syncthreads();
+    if (wv != 0) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+#pragma unroll
+            for (int j = 0; j < CW; ++j) atomicAdd(&red[lane * LSTR + w * CW + j], tot[w][j]);
+        }
+    }
+    __syncthreads();
+    const int64_t cbase = static_cast<int64_t>(blockIdx.x) * 64 * CPU;
+    uint32_t* dst = part + static_cast<int64_t>(blockIdx.y) * nchans;
+    for (int l = threadIdx.x; l < 64 * CPU; l += 256) {
+        const int64_t c = cbase + l;
+        if (c < nchans) dst[c] = red[(l / CPU) * LSTR + l % CPU];
+    }
+}
+
+// Bandpass, pass 2: one channel per lane, 16 waves over the chunks. T = the gulp's exact sum.
+//   parallel (kernels.extract_bpass_par): out = float32(float64(out) + T), numba's
+//     float32 + float64 -> float64 store;
+//   sequential (kernels.extract_bpass): out + T when every running float32 sum stays an
+//     integer within 2^24 (then it is exact); otherwise the exact prefix up to the first chunk
+//     that could leave that range, then float32 adds in time order from there.
+constexpr int BPF_WAVES = 16;
 template <int NBITS, bool BIG, bool SEQ>
-__global__ __launch_bounds__(256) void k_bandpass(const void* __restrict__ in, int64_t nchans,
-                                                  int64_t nsamps, int64_t row_bytes,
-                                                  float* __restrict__ out) {
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (c >= nchans) return;
-    if constexpr (SEQ) {
-        float acc = out[c];
-        int64_t t = 0;
+__global__ __launch_bounds__(64 * BPF_WAVES) void k_bp_finish(const uint32_t* __restrict__ part,
+                                                              int64_t nchunks, int64_t chunk_rows,
+                                                              int64_t nchans, const uint8_t* __restrict__ in,
+                                                              int64_t nsamps, int64_t row_bytes,
+                                                              float* __restrict__ out) {
+    __shared__ uint64_t red[BPF_WAVES][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * 64 + lane;
+    uint64_t s = 0;
+    if (c < nchans) {
+#pragma unroll 8
+        for (int64_t j = wv; j < nchunks; j += BPF_WAVES) s += part[j * nchans + c];
+    }
+    red[wv][lane] = s;
+    __syncthreads();
+    if (wv != 0 || c >= nchans) return;
+    uint64_t tsum = 0;
+#pragma unroll
+    for (int k = 0; k < BPF_WAVES; ++k) tsum += red[k][lane];
+    const float o = out[c];
+    const double total = static_cast<double>(tsum);
+    if constexpr (!SEQ) {
+        out[c] = static_cast<float>(static_cast<double>(o) + total);
+    } else {
+        constexpr double LIM = 16777216.0;
+        const bool o_int = o == truncf(o) && fabs(static_cast<double>(o)) <= LIM;
+        if (o_int && fabs(static_cast<double>(o) + total) <= LIM) {
+            out[c] = static_cast<float>(static_cast<double>(o) + total);
+            return;
+        }
+        double pre = o;
+        int64_t j = 0;
+        if (o_int) {
+            while (j < nchunks) {
+                const double nx = pre + static_cast<double>(part[j * nchans + c]);
+                if (fabs(nx) > LIM) break;
+                pre = nx;
+                ++j;
+            }
+        }
+        float acc = static_cast<float>(pre);
+        int64_t t = j * chunk_rows;
         for (; t + 8 <= nsamps; t += 8) {
             float v[8];
 #pragma unroll
@@ -47,11 +296,59 @@ __global__ __launch_bounds__(256) void k_bandpass(const void* __restrict__ in, i
         }
         for (; t < nsamps; ++t) acc = acc + sample_at<NBITS, BIG>(in, t, row_bytes, c);
         out[c] = acc;
-    } else {
-        double acc = 0.0;
-        for (int64_t t = 0; t < nsamps; ++t) acc += static_cast<double>(sample_at<NBITS, BIG>(in, t, row_bytes, c));
-        out[c] = out[c] + static_cast<float>(acc);
     }
+}
+
+// Bandpass of float32 rows, one thread per channel, time in order.
+//   SEQ:  out[c] = (...((out[c] + x[0][c]) + x[1][c]) ...) in float32 (kernels.extract_bpass);
+//   !SEQ: out[c] = float32(float64(out[c]) + sum_t x[t][c] in float64, time order)
+//         (kernels.extract_bpass_par).
+template <bool SEQ>
+__global__ __launch_bounds__(256) void k_bandpass_f32(const float* __restrict__ in, int64_t nchans,
+                                                      int64_t nsamps, float* __restrict__ out) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (c >= nchans) return;
+    using acc_t = std::conditional_t<SEQ, float, double>;
+    acc_t acc = SEQ ? static_cast<acc_t>(out[c]) : acc_t(0);
+    int64_t t = 0;
+    for (; t + 16 <= nsamps; t += 16) {
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = in[(t + u) * nchans + c];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc = acc + static_cast<acc_t>(v[u]);
+    }
+    for (; t < nsamps; ++t) acc = acc + static_cast<acc_t>(in[t * nchans + c]);
+    if constexpr (SEQ) out[c] = acc;
+    else out[c] = static_cast<float>(static_cast<double>(out[c]) + acc);
+}
+
+// Pass-1 geometry shared by the workspace query and the launch: lane-unit width, grid
+// columns, chunk count and chunk length.
+struct BpGeom {
+    int vw;
+    int64_t gx, nchunks, chunk_rows;
+};
+
+BpGeom bp_geom(int nbits, int64_t row_bytes, int64_t nsamps, bool aligned16, bool aligned8,
+               bool aligned4) {
+    BpGeom g{};
+    const int vmax = nbits == 1 ? 2 : 4;
+    if (vmax == 4 && aligned16 && row_bytes % 16 == 0) g.vw = 4;
+    else if (vmax == 2 && aligned8 && row_bytes % 8 == 0) g.vw = 2;
+    else if (aligned4 && row_bytes % 4 == 0) g.vw = 1;
+    else g.vw = 0;
+    const int64_t ub = g.vw ? 4 * g.vw : 1;
+    g.gx = spp::ceil_div(row_bytes / ub, int64_t(64));
+    // The chunk count comes from the widest unit, so the workspace query (which does not
+    // know the alignment) and every launch agree on it.
+    const int64_t gx_min = spp::ceil_div(spp::ceil_div(row_bytes, int64_t(4 * vmax)), int64_t(64));
+    const int64_t want = std::max<int64_t>(1, spp::ceil_div(BP_BLOCKS, gx_min));
+    const int64_t maxc = std::max<int64_t>(1, spp::ceil_div(nsamps, int64_t(4 * BP_R)));
+    const int64_t nc = std::min(want, maxc);
+    g.chunk_rows = spp::ceil_div(spp::ceil_div(nsamps, nc), int64_t(4 * BP_R)) * (4 * BP_R);
+    g.nchunks = spp::ceil_div(nsamps, g.chunk_rows);
+    return g;
 }
 
 // Masking on the file's bytes: byte j of every row becomes (b & keep[j]) | set[j]. The host
@@ -108,23 +405,71 @@ int check_nbits(int nbits) {
 
 }  // namespace
 
+extern "C" int spp_bandpass_workspace_size(int nbits, int64_t nchans, int64_t nsamps, int64_t* bytes) {
+    spp::clear_error();
+    if (check_nbits(nbits)) return SPP_EINVAL;
+    SPP_REQUIRE(bytes != nullptr, "null argument");
+    SPP_REQUIRE(nchans > 0 && nsamps >= 0, "nchans must be > 0 and nsamps >= 0");
+    SPP_REQUIRE((nchans * nbits) % 8 == 0, "nchans * nbits must be a multiple of 8");
+    *bytes = 0;
+    if (nbits == 32 || nsamps == 0) return SPP_OK;
+    const BpGeom g = bp_geom(nbits, nchans * nbits / 8, nsamps, true, true, true);
+    *bytes = g.nchunks * nchans * static_cast<int64_t>(sizeof(uint32_t));
+    return SPP_OK;
+}
+
 extern "C" int spp_bandpass(const void* in, int nbits, int big, int64_t nchans, int64_t nsamps,
-                            float* out, int parallel, void* stream) {
+                            float* out, int parallel, void* workspace, int64_t workspace_bytes,
+                            void* stream) {
     spp::clear_error();
     if (check_nbits(nbits)) return SPP_EINVAL;
     SPP_REQUIRE(in != nullptr && out != nullptr, "null argument");
     SPP_REQUIRE(nchans > 0 && nsamps >= 0, "nchans must be > 0 and nsamps >= 0");
     SPP_REQUIRE((nchans * nbits) % 8 == 0, "nchans * nbits must be a multiple of 8");
+    SPP_REQUIRE(nsamps < (int64_t(1) << 31), "nsamps must be < 2^31");
     if (nsamps == 0) return SPP_OK;
     hipStream_t st = spp::as_stream(stream);
     const int64_t row_bytes = nchans * nbits / 8;
-    const dim3 grid(static_cast<unsigned>(spp::ceil_div(nchans, int64_t(256))));
+    if (nbits == 32) {
+        const dim3 grid(static_cast<unsigned>(spp::ceil_div(nchans, int64_t(256))));
+        if (parallel)
+            hipLaunchKernelGGL((k_bandpass_f32<false>), grid, dim3(256), 0, st, static_cast<const float*>(in),
+                               nchans, nsamps, out);
+        else
+            hipLaunchKernelGGL((k_bandpass_f32<true>), grid, dim3(256), 0, st, static_cast<const float*>(in),
+                               nchans, nsamps, out);
+        SPP_LAUNCHED("k_bandpass_f32");
+        return SPP_OK;
+    }
+    const uintptr_t a = reinterpret_cast<uintptr_t>(in);
+    const BpGeom g = bp_geom(nbits, row_bytes, nsamps, a % 16 == 0, a % 8 == 0, a % 4 == 0);
+    const int64_t need = g.nchunks * nchans * static_cast<int64_t>(sizeof(uint32_t));
+    SPP_REQUIRE(workspace != nullptr, "null workspace");
+    SPP_REQUIRE(workspace_bytes >= need, "workspace too small: %lld < %lld",
+                static_cast<long long>(workspace_bytes), static_cast<long long>(need));
+    SPP_REQUIRE(g.gx <= 0x7fffffffLL && g.nchunks <= 65535, "gulp too large");
+    auto* part = static_cast<uint32_t*>(workspace);
+    const auto* src = static_cast<const uint8_t*>(in);
+    const dim3 g1(static_cast<unsigned>(g.gx), static_cast<unsigned>(g.nchunks));
+    const dim3 g2(static_cast<unsigned>(spp::ceil_div(nchans, int64_t(64))));
+#define SPP_BP_PART(NB, BG, V)                                                                         \
+    hipLaunchKernelGGL((k_bp_part<NB, BG, V>), g1, dim3(256), 0, st, src, nsamps, row_bytes, g.chunk_rows, \
+                       nchans, part)
 #define SPP_BP(NB, BG)                                                                                   \
     do {                                                                                                 \
-        if (parallel) hipLaunchKernelGGL((k_bandpass<NB, BG, false>), grid, dim3(256), 0, st, in, nchans, \
-                                         nsamps, row_bytes, out);                                        \
-        else hipLaunchKernelGGL((k_bandpass<NB, BG, true>), grid, dim3(256), 0, st, in, nchans, nsamps,  \
-                                row_bytes, out);                                                         \
+        switch (g.vw) {                                                                                  \
+            case 4: SPP_BP_PART(NB, BG, 4); break;                                                       \
+            case 2: SPP_BP_PART(NB, BG, 2); break;                                                       \
+            case 1: SPP_BP_PART(NB, BG, 1); break;                                                       \
+            default: SPP_BP_PART(NB, BG, 0); break;                                                      \
+        }                                                                                                \
+        SPP_LAUNCHED("k_bp_part");                                                                       \
+        if (parallel)                                                                                    \
+            hipLaunchKernelGGL((k_bp_finish<NB, BG, false>), g2, dim3(64 * BPF_WAVES), 0, st, part,      \
+                               g.nchunks, g.chunk_rows, nchans, src, nsamps, row_bytes, out);            \
+        else                                                                                             \
+            hipLaunchKernelGGL((k_bp_finish<NB, BG, true>), g2, dim3(64 * BPF_WAVES), 0, st, part,       \
+                               g.nchunks, g.chunk_rows, nchans, src, nsamps, row_bytes, out);            \
     } while (0)
     switch (nbits * 2 + (big ? 1 : 0)) {
         case 2: SPP_BP(1, false); break;
@@ -133,11 +478,52 @@ extern "C" int spp_bandpass(const void* in, int nbits, int big, int64_t nchans, 
         case 5: SPP_BP(2, true); break;
         case 8: SPP_BP(4, false); break;
         case 9: SPP_BP(4, true); break;
-        case 16: case 17: SPP_BP(8, false); break;
-        default: SPP_BP(32, false); break;
+        default: SPP_BP(8, false); break;
     }
 #undef SPP_BP
-    SPP_LAUNCHED("k_bandpass");
+#undef SPP_BP_PART
+    SPP_LAUNCHED("k_bp_finish");
+    return SPP_OK;
+}
+
+extern "C" int spp_collapse(const void* in, int nbits, int big, int64_t nchans, int64_t nsamps,
+                            float* out, int64_t out_start, void* stream) {
+    spp::clear_error();
+    (void)big;  // a sum over every field of a row does not depend on the bit order
+    if (check_nbits(nbits)) return SPP_EINVAL;
+    SPP_REQUIRE(in != nullptr && out != nullptr, "null argument");
+    SPP_REQUIRE(nchans > 0 && nsamps >= 0 && out_start >= 0, "nchans must be > 0, nsamps and out_start >= 0");
+    SPP_REQUIRE((nchans * nbits) % 8 == 0, "nchans * nbits must be a multiple of 8");
+    SPP_REQUIRE(nbits == 32 || nchans < (int64_t(1) << 24), "nchans must be < 2^24");
+    if (nsamps == 0) return SPP_OK;
+    hipStream_t st = spp::as_stream(stream);
+    const int64_t row_bytes = nchans * nbits / 8;
+    float* o = out + out_start;
+    if (nbits == 32) {
+        hipLaunchKernelGGL(k_collapse_f32, dim3(static_cast<unsigned>(spp::ceil_div(nsamps, int64_t(256)))),
+                           dim3(256), 0, st, static_cast<const float*>(in), nsamps, nchans, o);
+        SPP_LAUNCHED("k_collapse_f32");
+        return SPP_OK;
+    }
+    const uintptr_t a = reinterpret_cast<uintptr_t>(in);
+    const int ub = (row_bytes % 16 == 0 && a % 16 == 0) ? 16 : (row_bytes % 4 == 0 && a % 4 == 0) ? 4 : 1;
+    const int64_t blocks = spp::ceil_div(nsamps, int64_t(4 * CL_ROWS));
+    SPP_REQUIRE(blocks <= 0x7fffffffLL, "gulp too long");
+    const auto* src = static_cast<const uint8_t*>(in);
+#define SPP_CL(NB)                                                                                      \
+    do {                                                                                                \
+        if (ub == 16) hipLaunchKernelGGL((k_collapse<NB, 16>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, src, nsamps, row_bytes, o); \
+        else if (ub == 4) hipLaunchKernelGGL((k_collapse<NB, 4>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, src, nsamps, row_bytes, o); \
+        else hipLaunchKernelGGL((k_collapse<NB, 1>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, src, nsamps, row_bytes, o); \
+    } while (0)
+    switch (nbits) {
+        case 1: SPP_CL(1); break;
+        case 2: SPP_CL(2); break;
+        case 4: SPP_CL(4); break;
+        default: SPP_CL(8); break;
+    }
+#undef SPP_CL
+    SPP_LAUNCHED("k_collapse");
     return SPP_OK;
 }
 

@@ -346,6 +346,27 @@ def test_bandpass_sequential_float32_and_exact_integers(rng):
                                   exact.astype(np.float32) / np.float32(3000))
 
 
+def test_bandpass_parallel_rounds_once_per_gulp():
+    """extract_bpass_par adds a float64 gulp sum to the float32 bandpass in float64 and rounds
+    once. One channel, gulps of 65 800 samples: gulp 1 sums to 1, gulp 2 to 2^24 + 1. Rounded
+    once, 1 + 2^24 + 1 = 2^24 + 2 is representable; rounding the gulp sum to float32 first
+    (2^24, a tie to even) and then adding 1 (another tie) would give 2^24."""
+    g = 65800
+    x = np.zeros(2 * g, dtype=np.uint8)
+    x[0] = 1
+    x[g : g + 65793] = 255                    # 65793 * 255 = 2^24 - 1
+    x[g + 65793] = 2
+    assert int(x[g:].astype(np.int64).sum()) == 2**24 + 1
+    got = O.bandpass(x, 2 * g, 1, gulp=g, parallel=True)
+    assert got[0] == np.float32(2**24 + 2)
+    # the sequential float32 order: 1, then 2^24 - 1 more reaches 2^24 exactly, and + 2 -> 2^24 + 2
+    seq = O.bandpass(x, 2 * g, 1, gulp=g)
+    acc = np.float32(0)
+    for v in x:                              # brute force, scalar float32 adds
+        acc = np.float32(acc + np.float32(v))
+    assert seq[0] == acc
+
+
 def test_fma_exact_single_rounding():
     a = 1.0 + 2.0**-30
     # a*a = 1 + 2^-29 + 2^-60: two roundings lose the 2^-60 term, one rounding keeps it
