@@ -13,6 +13,7 @@
 // Work per output sample: nchans adds; HBM bytes per gulp: nsamps * nchans * nbits/8
 // read + 4 * nout written. One DM per launch is HBM-bound.
 #include "spp_common.hpp"
+#include "spp_unpack.hpp"
 
 #include <algorithm>
 #include <type_traits>
@@ -92,7 +93,12 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
         }
         __syncthreads();
         int dmin, dmax;
-        const int n = stage_extent(sdel, ncand, DD_ROWS - DD_T, dmin, dmax);
+        int n = stage_extent(sdel, ncand, DD_ROWS - DD_T, dmin, dmax);
+        if (n < ncand && n >= 4) {
+            // Keep stage starts on multiples of 4 channels (dword-aligned tile columns for the
+            // word unpack below); a prefix of a stage fits the rows too. dmin/dmax stay bounds.
+            n &= ~3;
+        }
         const int nrows = DD_T + (dmax - dmin);
         const int64_t rbase = t0 + dmin;
         {
@@ -102,12 +108,16 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
             const int64_t ub0 = bb0 / UB * UB;  // first unit's byte
             const int nunit = static_cast<int>((bb1 - ub0 + UB - 1) / UB);
             const int nel = nrows * nunit;
+            // e / nunit as a multiply-high: exact while e * nunit < 2^32 (e < 768 * 65 here)
+            const uint32_t magic = nunit > 1 ? static_cast<uint32_t>(0xffffffffu / static_cast<uint32_t>(nunit) + 1u) : 0u;
+            const int col_base = static_cast<int>(ub0 * PER - c);  // tile column of unit 0's first sample
             for (int e0 = 0; e0 < nel; e0 += DD_LD * DD_THREADS) {
                 uint32_t v[DD_LD];
 #pragma unroll
                 for (int u = 0; u < DD_LD; ++u) {
                     const int e = e0 + u * DD_THREADS + tid;
-                    const int r = e / nunit, q = e - r * nunit;
+                    const int r = nunit > 1 ? static_cast<int>(__umulhi(static_cast<uint32_t>(e), magic)) : e;
+                    const int q = e - r * nunit;
                     int64_t row = rbase + r;
                     if (row < 0) row += nsamps;  // numba wraparound of a negative index
                     v[u] = 0;
@@ -120,14 +130,26 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
                 for (int u = 0; u < DD_LD; ++u) {
                     const int e = e0 + u * DD_THREADS + tid;
                     if (e >= nel) break;
-                    const int r = e / nunit, q = e - r * nunit;
+                    const int r = nunit > 1 ? static_cast<int>(__umulhi(static_cast<uint32_t>(e), magic)) : e;
+                    const int q = e - r * nunit;
+                    const int col0 = col_base + UB * PER * q;  // tile column of the unit's first sample
+                    if constexpr (WIDE) {
+                        if (col0 >= 0 && col0 + 4 * PER <= n && (col0 & 3) == 0) {
+                            // interior dword: all 4 * PER samples land in the tile, dword-aligned
+                            uint32_t o[PER];
+                            spp::expand_samples<NBITS, BIG>(v[u], o);
+                            uint32_t* dst = reinterpret_cast<uint32_t*>(tile + r * STRIDE + col0);
+#pragma unroll
+                            for (int j = 0; j < PER; ++j) dst[j] = o[j];
+                            continue;
+                        }
+                    }
 #pragma unroll
                     for (int jb = 0; jb < UB; ++jb) {
                         const uint32_t byte = (v[u] >> (8 * jb)) & 0xffu;
-                        const int64_t ch0 = (ub0 + UB * q + jb) * PER;  // first channel of the byte
 #pragma unroll
                         for (int k = 0; k < PER; ++k) {
-                            const int64_t ch = ch0 + k - c;
+                            const int ch = col0 + jb * PER + k;
                             if (ch >= 0 && ch < n) {
                                 const int shift = (NBITS == 8) ? 0 : (BIG ? (8 - (k + 1) * NBITS) : k * NBITS);
                                 tile[r * STRIDE + ch] = static_cast<uint8_t>((byte >> shift) & ((1u << NBITS) - 1u));

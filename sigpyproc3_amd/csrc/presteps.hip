@@ -4,6 +4,7 @@
 // Gulp layout: file order [nsamps][nchans], packed nbits (1/2/4/8, bit order `big`) or
 // float32 (nbits == 32). All three are HBM-bound single passes over the gulp.
 #include "spp_common.hpp"
+#include "spp_unpack.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -397,6 +398,116 @@ __global__ __launch_bounds__(256) void k_zerodm(const void* __restrict__ in, int
     }
 }
 
+// Masking with dword or 16-byte units: a thread keeps one unit's keep/set bytes and applies
+// them to that unit of rows blockIdx.y, blockIdx.y + gridDim.y, ...
+template <int UB>
+__global__ __launch_bounds__(256) void k_mask_rows(uint8_t* __restrict__ data, int64_t nrows, int64_t row_bytes,
+                                                   const uint8_t* __restrict__ keep,
+                                                   const uint8_t* __restrict__ set) {
+    using unit_t = std::conditional_t<UB == 16, uint4, uint32_t>;
+    const int64_t nunit = row_bytes / UB;
+    const int64_t u = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (u >= nunit) return;
+    const unit_t k = reinterpret_cast<const unit_t*>(keep)[u];
+    const unit_t st = reinterpret_cast<const unit_t*>(set)[u];
+    constexpr int R = 4;
+    for (int64_t r0 = blockIdx.y; r0 < nrows; r0 += static_cast<int64_t>(gridDim.y) * R) {
+        unit_t v[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int64_t r = r0 + static_cast<int64_t>(i) * gridDim.y;
+            if (r < nrows) v[i] = reinterpret_cast<const unit_t*>(data + r * row_bytes)[u];
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int64_t r = r0 + static_cast<int64_t>(i) * gridDim.y;
+            if (r >= nrows) break;
+            unit_t o;
+            if constexpr (UB == 16) {
+                o = make_uint4((v[i].x & k.x) | st.x, (v[i].y & k.y) | st.y, (v[i].z & k.z) | st.z,
+                               (v[i].w & k.w) | st.w);
+            } else {
+                o = (v[i] & k) | st;
+            }
+            reinterpret_cast<unit_t*>(data + r * row_bytes)[u] = o;
+        }
+    }
+}
+
+// Zero-DM removal of integer data with dword units (row bytes and pointers 4-byte aligned):
+// a thread owns UPT dwords of every row (4 * 8/NBITS channels each, at most 32 channels) and
+// keeps their weights and bandpass values in registers; the block walks rows. Per row: z from the
+// exact field sums (block reduction), then every sample as in k_zerodm, four output bytes per
+// dword store.
+template <int NBITS, bool BIG, int UPT>
+__global__ __launch_bounds__(256) void k_zerodm_w(const uint8_t* __restrict__ in, int64_t nsamps,
+                                                  int64_t nchans, int64_t row_bytes,
+                                                  const float* __restrict__ w, const float* __restrict__ bpass,
+                                                  uint8_t* __restrict__ out) {
+    constexpr int PER = 8 / NBITS;
+    constexpr int CPW = 4 * PER;  // channels per dword
+    __shared__ uint32_t zpart[2][4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t nunit = row_bytes / 4;
+    static_assert(UPT * CPW <= 32, "channels per thread");
+    float wf[UPT][CPW], bf[UPT][CPW];
+#pragma unroll
+    for (int i = 0; i < UPT; ++i) {
+        const int64_t u = tid + 256 * i;
+#pragma unroll
+        for (int j = 0; j < CPW; ++j) {
+            const int64_t c = u * CPW + j;
+            wf[i][j] = u < nunit ? w[c] : 0.f;
+            bf[i][j] = u < nunit ? bpass[c] : 0.f;
+        }
+    }
+    auto load_row = [&](int64_t t, uint32_t (&v)[UPT]) {
+#pragma unroll
+        for (int i = 0; i < UPT; ++i) {
+            const int64_t u = tid + 256 * i;
+            v[i] = (t < nsamps && u < nunit) ? reinterpret_cast<const uint32_t*>(in + t * row_bytes)[u] : 0u;
+        }
+    };
+    int par = 0;
+    uint32_t v[UPT];
+    load_row(blockIdx.x, v);
+    for (int64_t t = blockIdx.x; t < nsamps; t += gridDim.x) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int i = 0; i < UPT; ++i) s = field_sum<NBITS>(v[i], s);
+        s = wave_sum(s);
+        if (lane == 0) zpart[par][wv] = s;
+        __syncthreads();
+        const double z = static_cast<double>(zpart[par][0] + zpart[par][1] + zpart[par][2] + zpart[par][3]);
+        par ^= 1;
+        uint32_t cur[UPT];
+#pragma unroll
+        for (int i = 0; i < UPT; ++i) cur[i] = v[i];
+        load_row(t + gridDim.x, v);  // next row in flight while this one is written
+#pragma unroll
+        for (int i = 0; i < UPT; ++i) {
+            const int64_t u = tid + 256 * i;
+            if (u >= nunit) break;
+            uint32_t o[PER];
+            spp::expand_samples<NBITS, BIG>(cur[i], o);
+            uint32_t* dst = reinterpret_cast<uint32_t*>(out + t * nchans + u * CPW);
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                uint32_t res = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const double x = static_cast<double>((o[j] >> (8 * k)) & 0xffu);
+                    const double y = fma(-z, static_cast<double>(wf[i][4 * j + k]), x) +
+                                     static_cast<double>(bf[i][4 * j + k]);
+                    const double yc = fmin(fmax(y, -2147483648.0), 2147483647.0);
+                    res |= (static_cast<uint32_t>(static_cast<int32_t>(yc)) & 0xffu) << (8 * k);
+                }
+                dst[j] = res;
+            }
+        }
+    }
+}
+
 int check_nbits(int nbits) {
     SPP_REQUIRE(nbits == 1 || nbits == 2 || nbits == 4 || nbits == 8 || nbits == 32,
                 "nbits must be 1, 2, 4, 8 or 32, got %d", nbits);
@@ -535,9 +646,24 @@ extern "C" int spp_mask_bytes(uint8_t* data, int64_t nbytes, int64_t row_bytes, 
                 "nbytes (%lld) must be a multiple of row_bytes (%lld)", static_cast<long long>(nbytes),
                 static_cast<long long>(row_bytes));
     if (nbytes == 0) return SPP_OK;
+    hipStream_t st = spp::as_stream(stream);
+    const uintptr_t align = reinterpret_cast<uintptr_t>(data) | reinterpret_cast<uintptr_t>(keep) |
+                            reinterpret_cast<uintptr_t>(set);
+    const int ub = (row_bytes % 16 == 0 && align % 16 == 0) ? 16 : (row_bytes % 4 == 0 && align % 4 == 0) ? 4 : 0;
+    if (ub) {
+        const int64_t nrows = nbytes / row_bytes;
+        const int64_t gx = spp::ceil_div(row_bytes / ub, int64_t(256));
+        const int64_t gy = std::max<int64_t>(1, std::min<int64_t>({nrows, spp::ceil_div(2048, gx), 65535}));
+        SPP_REQUIRE(gx <= 0x7fffffffLL, "rows too long");
+        const dim3 grid(static_cast<unsigned>(gx), static_cast<unsigned>(gy));
+        if (ub == 16) hipLaunchKernelGGL((k_mask_rows<16>), grid, dim3(256), 0, st, data, nrows, row_bytes, keep, set);
+        else hipLaunchKernelGGL((k_mask_rows<4>), grid, dim3(256), 0, st, data, nrows, row_bytes, keep, set);
+        SPP_LAUNCHED("k_mask_rows");
+        return SPP_OK;
+    }
     const int64_t blocks = std::min<int64_t>(spp::ceil_div(nbytes, int64_t(256)), 65536);
-    hipLaunchKernelGGL(k_mask_bytes, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
-                       spp::as_stream(stream), data, nbytes, row_bytes, keep, set);
+    hipLaunchKernelGGL(k_mask_bytes, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, data, nbytes,
+                       row_bytes, keep, set);
     SPP_LAUNCHED("k_mask_bytes");
     return SPP_OK;
 }
@@ -553,6 +679,46 @@ extern "C" int spp_remove_zerodm(const void* in, int nbits, int big, int64_t nch
     if (nsamps == 0) return SPP_OK;
     hipStream_t st = spp::as_stream(stream);
     const int64_t row_bytes = nchans * nbits / 8;
+    const uintptr_t align = reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out);
+    int64_t upt = spp::ceil_div(row_bytes / 4, int64_t(256));
+    upt = upt <= 1 ? 1 : upt <= 2 ? 2 : upt <= 4 ? 4 : upt <= 8 ? 8 : upt;
+    const int64_t cpw = nbits == 32 ? 1 : 32 / nbits;  // channels per dword
+    if (nbits != 32 && row_bytes % 4 == 0 && align % 4 == 0 && upt * cpw <= 32) {
+        const dim3 grid(static_cast<unsigned>(std::min<int64_t>(nsamps, 4 * spp::device_cu_count())));
+        const auto* src = static_cast<const uint8_t*>(in);
+        auto* dst = static_cast<uint8_t*>(out);
+#define SPP_ZDW(NB, BG, U) \
+    hipLaunchKernelGGL((k_zerodm_w<NB, BG, U>), grid, dim3(256), 0, st, src, nsamps, nchans, row_bytes, chanwts, bpass, dst)
+#define SPP_ZDW_U(NB, BG)                                              \
+    do {                                                               \
+        if constexpr (32 / NB >= 32) SPP_ZDW(NB, BG, 1);               \
+        else if constexpr (32 / NB >= 16) {                            \
+            if (upt == 1) SPP_ZDW(NB, BG, 1); else SPP_ZDW(NB, BG, 2); \
+        } else if constexpr (32 / NB >= 8) {                           \
+            if (upt == 1) SPP_ZDW(NB, BG, 1);                          \
+            else if (upt == 2) SPP_ZDW(NB, BG, 2);                     \
+            else SPP_ZDW(NB, BG, 4);                                   \
+        } else {                                                       \
+            if (upt == 1) SPP_ZDW(NB, BG, 1);                          \
+            else if (upt == 2) SPP_ZDW(NB, BG, 2);                     \
+            else if (upt == 4) SPP_ZDW(NB, BG, 4);                     \
+            else SPP_ZDW(NB, BG, 8);                                   \
+        }                                                              \
+    } while (0)
+        switch (nbits * 2 + (big ? 1 : 0)) {
+            case 2: SPP_ZDW_U(1, false); break;
+            case 3: SPP_ZDW_U(1, true); break;
+            case 4: SPP_ZDW_U(2, false); break;
+            case 5: SPP_ZDW_U(2, true); break;
+            case 8: SPP_ZDW_U(4, false); break;
+            case 9: SPP_ZDW_U(4, true); break;
+            default: SPP_ZDW_U(8, false); break;
+        }
+#undef SPP_ZDW_U
+#undef SPP_ZDW
+        SPP_LAUNCHED("k_zerodm_w");
+        return SPP_OK;
+    }
     const dim3 grid(static_cast<unsigned>(nsamps));
 #define SPP_ZD(NB, BG) \
     hipLaunchKernelGGL((k_zerodm<NB, BG>), grid, dim3(256), 0, st, in, nchans, row_bytes, chanwts, bpass, out)

@@ -198,3 +198,61 @@ def _band_starts(nchans: int, nbands: int) -> np.ndarray:
     from sigpyproc3_amd.foldedcube import band_starts
 
     return band_starts(nchans, nbands)
+
+
+def _zerodm(src, nbits, big, nchans, nsamps, bpass, out_offset=0):
+    from sigpyproc3_amd import _lib
+
+    w_d = _lib.to_device((bpass / bpass.sum()).astype(np.float32))
+    b_d = _lib.to_device(bpass.astype(np.float32))
+    esz = 4 if nbits == 32 else 1
+    out = _lib.zeros((nsamps * nchans * esz + out_offset,), "uint8")
+    view = out[out_offset:]
+    _lib.call("spp_remove_zerodm", _lib.ptr(src), nbits, big, nchans, nsamps, _lib.ptr(w_d), _lib.ptr(b_d),
+              _lib.ptr(view), _lib.stream_handle())
+    res = view.cpu().numpy()
+    return res.view(np.float32) if nbits == 32 else res
+
+
+@pytest.mark.parametrize("nbits", [1, 2, 4, 8])
+@pytest.mark.parametrize("order", ["big", "little"])
+def test_remove_zerodm_register_path(nbits, order, rng):
+    """The dword path (weights in registers, 1-8 dwords per thread) against the exact fused
+    oracle on small rows, and against the per-row byte kernel (misaligned output) at widths
+    up to 8192 channels."""
+    big = 1 if order == "big" else 0
+    nchans, nsamps = 64 * 8 // nbits, 40
+    x = rng.integers(0, 1 << nbits, (nsamps, nchans), dtype=np.uint8)
+    bpass = rng.uniform(1.0e3, 5.0e4, nchans).astype(np.float32)
+    _keep, src = _dev(_packed(x, nbits, order))
+    exp = O.remove_zerodm(x.reshape(-1), nchans, bpass, np.uint8, fused=True)
+    np.testing.assert_array_equal(_zerodm(src, nbits, big, nchans, nsamps, bpass), exp)
+    for nchans in (1024, 4096, 8192 if nbits == 8 else 2048, 1040):
+        nsamps = 50
+        x = rng.integers(0, 1 << nbits, (nsamps, nchans), dtype=np.uint8)
+        bpass = rng.uniform(1.0e3, 5.0e4, nchans).astype(np.float32)
+        _keep, src = _dev(_packed(x, nbits, order))
+        fast = _zerodm(src, nbits, big, nchans, nsamps, bpass)
+        slow = _zerodm(src, nbits, big, nchans, nsamps, bpass, out_offset=1)  # byte kernel
+        np.testing.assert_array_equal(fast, slow, err_msg=f"{nchans}")
+        # z is exact and y = x - z*w + b lands within a rounding of the unfused form
+        ref = O.remove_zerodm(x.reshape(-1), nchans, bpass, np.uint8, fused=False)
+        assert np.mean(fast == ref) > 0.99
+
+
+@pytest.mark.parametrize("row_bytes, offset", [(4096, 0), (1028, 4), (77, 0), (512, 3)])
+def test_mask_bytes_unit_widths(row_bytes, offset, rng):
+    from sigpyproc3_amd import _lib
+
+    nrows = 333
+    data = rng.integers(0, 256, nrows * row_bytes, dtype=np.uint8)
+    keep = rng.integers(0, 256, row_bytes, dtype=np.uint8)
+    setb = rng.integers(0, 256, row_bytes, dtype=np.uint8) & ~keep
+    _kd, dd = _dev(data, offset)
+    _kk, kd = _dev(keep, offset)
+    _ks, sd = _dev(setb, offset)
+    _lib.call("spp_mask_bytes", _lib.ptr(dd), data.size, row_bytes, _lib.ptr(kd), _lib.ptr(sd),
+              _lib.stream_handle())
+    got = dd[: data.size].cpu().numpy()
+    exp = ((data.reshape(nrows, row_bytes) & keep) | setb).reshape(-1)
+    np.testing.assert_array_equal(got, exp)
