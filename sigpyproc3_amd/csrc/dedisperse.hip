@@ -24,25 +24,32 @@ constexpr int DD_THREADS = 256;
 constexpr int DD_T = 512;       // output samples per block (2 per thread)
 constexpr int DD_CC = 64;       // channels per stage at most
 constexpr int DD_ROWS = 768;    // staged rows per stage: spread <= DD_ROWS - DD_T
-constexpr int DD_LD = 16;       // staging loads in flight per thread
+constexpr int DD_LD = 8;        // staging loads in flight per thread
 constexpr int DDF_CC = 16;      // float32 path: channels per stage
 constexpr int DDF_ROWS = 768;
 constexpr int DDF_STRIDE = 17;  // floats per staged row
 
-// Greedy stage: the longest prefix of sdel[0..ncand) whose spread <= max_spread.
+// Greedy stage: the longest prefix of sdel[0..ncand) (ncand <= 64) whose delay spread is at
+// most max_spread, with its minimum and maximum delay. Every wave computes it: an inclusive
+// scan of min and max across the lanes, and since the prefix spreads only grow, the lanes
+// whose prefix fits are lanes 0 .. n-1.
 __device__ __forceinline__ int stage_extent(const int32_t* sdel, int ncand, int max_spread,
                                             int& dmin, int& dmax) {
-    dmin = sdel[0];
-    dmax = sdel[0];
-    int n = 1;
-    while (n < ncand) {
-        const int d = sdel[n];
-        const int lo = min(dmin, d), hi = max(dmax, d);
-        if (hi - lo > max_spread) break;
-        dmin = lo;
-        dmax = hi;
-        ++n;
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const int d = sdel[lane < ncand ? lane : 0];
+    int mn = d, mx = d;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int a = __shfl_up(mn, off), b = __shfl_up(mx, off);
+        if (lane >= off) {
+            mn = min(mn, a);
+            mx = max(mx, b);
+        }
     }
+    const bool ok = lane < ncand && mx - mn <= max_spread;
+    const int n = __popcll(__ballot(ok));
+    dmin = __shfl(mn, n - 1);
+    dmax = __shfl(mx, n - 1);
     return n;
 }
 
@@ -63,8 +70,8 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
     constexpr int STRIDE = CCX + 4;
     static_assert((STRIDE / 4) % 2 == 1 && STRIDE % 4 == 0, "odd dword row stride");
     __shared__ uint8_t tile[DD_ROWS * STRIDE];
-    __shared__ int32_t sdel[CCX];
-    __shared__ int32_t ssub[CCX];
+    __shared__ __align__(16) int32_t sdel[CCX];
+    __shared__ __align__(16) int32_t ssub[CCX];
     const int tid = threadIdx.x;
     const int64_t t0 = static_cast<int64_t>(blockIdx.x) * DD_T;
     const int64_t cb = static_cast<int64_t>(blockIdx.y) * chans_per_group;
@@ -111,6 +118,11 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
             // e / nunit as a multiply-high: exact while e * nunit < 2^32 (e < 768 * 65 here)
             const uint32_t magic = nunit > 1 ? static_cast<uint32_t>(0xffffffffu / static_cast<uint32_t>(nunit) + 1u) : 0u;
             const int col_base = static_cast<int>(ub0 * PER - c);  // tile column of unit 0's first sample
+            // Rows inside the gulp with 32-bit offsets: plain loads from one stage base.
+            const bool inrange = rbase >= 0 && rbase + nrows <= nsamps &&
+                                 static_cast<int64_t>(nrows) * row_bytes < (int64_t(1) << 31);
+            const uint8_t* sbase = in + (inrange ? rbase * row_bytes : 0) + ub0;
+            const int rb32 = static_cast<int>(inrange ? row_bytes : 0);
             for (int e0 = 0; e0 < nel; e0 += DD_LD * DD_THREADS) {
                 uint32_t v[DD_LD];
 #pragma unroll
@@ -118,12 +130,19 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
                     const int e = e0 + u * DD_THREADS + tid;
                     const int r = nunit > 1 ? static_cast<int>(__umulhi(static_cast<uint32_t>(e), magic)) : e;
                     const int q = e - r * nunit;
-                    int64_t row = rbase + r;
-                    if (row < 0) row += nsamps;  // numba wraparound of a negative index
                     v[u] = 0;
-                    if (e < nel && row >= 0 && row < nsamps) {
-                        const uint8_t* src = in + row * row_bytes + ub0 + UB * q;
-                        v[u] = WIDE ? *reinterpret_cast<const uint32_t*>(src) : *src;
+                    if (inrange) {
+                        if (e < nel) {
+                            const uint8_t* src = sbase + (r * rb32 + UB * q);
+                            v[u] = WIDE ? *reinterpret_cast<const uint32_t*>(src) : *src;
+                        }
+                    } else {
+                        int64_t row = rbase + r;
+                        if (row < 0) row += nsamps;  // numba wraparound of a negative index
+                        if (e < nel && row >= 0 && row < nsamps) {
+                            const uint8_t* src = in + row * row_bytes + ub0 + UB * q;
+                            v[u] = WIDE ? *reinterpret_cast<const uint32_t*>(src) : *src;
+                        }
                     }
                 }
 #pragma unroll
@@ -160,17 +179,51 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
             }
         }
         __syncthreads();
-        for (int k = 0; k < n; ++k) {
+        // Channels in steps of 8 (delays read as two 16-byte broadcasts, 16 tile reads in
+        // flight) while the 8 share the current subband; single channels otherwise. The step
+        // choice depends only on LDS values every thread sees, so it is uniform.
+        const int base0 = (tid - dmin) * STRIDE;  // + d * STRIDE lands in rows [tid, tid + spread]
+        const int base1 = base0 + DD_THREADS * STRIDE;
+        int k = 0;
+        while (k < n) {
+            bool step8 = k + 8 <= n;
             if constexpr (MODE == 1) {
-                const int sub = ssub[k];
-                if (sub != cur_sub) {
-                    flush_sub(cur_sub);
-                    cur_sub = sub;
+                if (step8) {
+                    const int4 s0 = *reinterpret_cast<const int4*>(ssub + k);
+                    const int4 s1 = *reinterpret_cast<const int4*>(ssub + k + 4);
+                    step8 = s0.x == cur_sub && s0.y == cur_sub && s0.z == cur_sub && s0.w == cur_sub &&
+                            s1.x == cur_sub && s1.y == cur_sub && s1.z == cur_sub && s1.w == cur_sub;
                 }
             }
-            const int rel = sdel[k] - dmin;
-            acc0 += tile[(tid + rel) * STRIDE + k];
-            acc1 += tile[(tid + DD_THREADS + rel) * STRIDE + k];
+            if (step8) {
+                const int4 d0 = *reinterpret_cast<const int4*>(sdel + k);
+                const int4 d1 = *reinterpret_cast<const int4*>(sdel + k + 4);
+                const int dd[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+                uint32_t a[8], b[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    a[j] = tile[base0 + dd[j] * STRIDE + k + j];
+                    b[j] = tile[base1 + dd[j] * STRIDE + k + j];
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    acc0 += a[j];
+                    acc1 += b[j];
+                }
+                k += 8;
+            } else {
+                if constexpr (MODE == 1) {
+                    const int sub = ssub[k];
+                    if (sub != cur_sub) {
+                        flush_sub(cur_sub);
+                        cur_sub = sub;
+                    }
+                }
+                const int d = sdel[k];
+                acc0 += tile[base0 + d * STRIDE + k];
+                acc1 += tile[base1 + d * STRIDE + k];
+                ++k;
+            }
         }
         c += n;
     }
