@@ -72,6 +72,7 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
     __shared__ uint8_t tile[DD_ROWS * STRIDE];
     __shared__ __align__(16) int32_t sdel[CCX];
     __shared__ __align__(16) int32_t ssub[CCX];
+    __shared__ __align__(16) int32_t soff[CCX];  // tile offset of channel k's column at row rel
     const int tid = threadIdx.x;
     const int64_t t0 = static_cast<int64_t>(blockIdx.x) * DD_T;
     const int64_t cb = static_cast<int64_t>(blockIdx.y) * chans_per_group;
@@ -108,81 +109,99 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
         }
         const int nrows = DD_T + (dmax - dmin);
         const int64_t rbase = t0 + dmin;
+        if (tid < n) soff[tid] = (sdel[tid] - dmin) * STRIDE + tid;  // read after the staging barrier
         {
             constexpr int PER = 8 / NBITS;  // samples per byte
             constexpr int UB = WIDE ? 4 : 1;  // bytes per unit
             const int64_t bb0 = c * NBITS / 8, bb1 = ((c + n) * NBITS + 7) / 8;
             const int64_t ub0 = bb0 / UB * UB;  // first unit's byte
             const int nunit = static_cast<int>((bb1 - ub0 + UB - 1) / UB);
-            const int nel = nrows * nunit;
-            // e / nunit as a multiply-high: exact while e * nunit < 2^32 (e < 768 * 65 here)
-            const uint32_t magic = nunit > 1 ? static_cast<uint32_t>(0xffffffffu / static_cast<uint32_t>(nunit) + 1u) : 0u;
             const int col_base = static_cast<int>(ub0 * PER - c);  // tile column of unit 0's first sample
-            // Rows inside the gulp with 32-bit offsets: plain loads from one stage base.
-            const bool inrange = rbase >= 0 && rbase + nrows <= nsamps &&
-                                 static_cast<int64_t>(nrows) * row_bytes < (int64_t(1) << 31);
-            const uint8_t* sbase = in + (inrange ? rbase * row_bytes : 0) + ub0;
-            const int rb32 = static_cast<int>(inrange ? row_bytes : 0);
-            for (int e0 = 0; e0 < nel; e0 += DD_LD * DD_THREADS) {
-                uint32_t v[DD_LD];
+            // One staged unit (row r of the stage, unit q): interior dwords expand to 4 * PER
+            // tile bytes with dword stores; edge units go byte by byte.
+            auto put = [&](uint32_t v, int r, int q) {
+                const int col0 = col_base + UB * PER * q;  // tile column of the unit's first sample
+                if constexpr (WIDE) {
+                    if (col0 >= 0 && col0 + 4 * PER <= n && (col0 & 3) == 0) {
+                        uint32_t o[PER];
+                        spp::expand_samples<NBITS, BIG>(v, o);
+                        uint32_t* dst = reinterpret_cast<uint32_t*>(tile + r * STRIDE + col0);
 #pragma unroll
-                for (int u = 0; u < DD_LD; ++u) {
-                    const int e = e0 + u * DD_THREADS + tid;
-                    const int r = nunit > 1 ? static_cast<int>(__umulhi(static_cast<uint32_t>(e), magic)) : e;
-                    const int q = e - r * nunit;
-                    v[u] = 0;
-                    if (inrange) {
-                        if (e < nel) {
-                            const uint8_t* src = sbase + (r * rb32 + UB * q);
-                            v[u] = WIDE ? *reinterpret_cast<const uint32_t*>(src) : *src;
-                        }
-                    } else {
-                        int64_t row = rbase + r;
-                        if (row < 0) row += nsamps;  // numba wraparound of a negative index
-                        if (e < nel && row >= 0 && row < nsamps) {
-                            const uint8_t* src = in + row * row_bytes + ub0 + UB * q;
-                            v[u] = WIDE ? *reinterpret_cast<const uint32_t*>(src) : *src;
-                        }
+                        for (int j = 0; j < PER; ++j) dst[j] = o[j];
+                        return;
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < DD_LD; ++u) {
-                    const int e = e0 + u * DD_THREADS + tid;
-                    if (e >= nel) break;
-                    const int r = nunit > 1 ? static_cast<int>(__umulhi(static_cast<uint32_t>(e), magic)) : e;
-                    const int q = e - r * nunit;
-                    const int col0 = col_base + UB * PER * q;  // tile column of the unit's first sample
-                    if constexpr (WIDE) {
-                        if (col0 >= 0 && col0 + 4 * PER <= n && (col0 & 3) == 0) {
-                            // interior dword: all 4 * PER samples land in the tile, dword-aligned
-                            uint32_t o[PER];
-                            spp::expand_samples<NBITS, BIG>(v[u], o);
-                            uint32_t* dst = reinterpret_cast<uint32_t*>(tile + r * STRIDE + col0);
+                for (int jb = 0; jb < UB; ++jb) {
+                    const uint32_t byte = (v >> (8 * jb)) & 0xffu;
 #pragma unroll
-                            for (int j = 0; j < PER; ++j) dst[j] = o[j];
-                            continue;
+                    for (int k = 0; k < PER; ++k) {
+                        const int ch = col0 + jb * PER + k;
+                        if (ch >= 0 && ch < n) {
+                            const int shift = (NBITS == 8) ? 0 : (BIG ? (8 - (k + 1) * NBITS) : k * NBITS);
+                            tile[r * STRIDE + ch] = static_cast<uint8_t>((byte >> shift) & ((1u << NBITS) - 1u));
                         }
                     }
+                }
+            };
+            auto load = [&](const uint8_t* src) -> uint32_t {
+                return WIDE ? *reinterpret_cast<const uint32_t*>(src) : static_cast<uint32_t>(*src);
+            };
+            // Rows inside the gulp (no wrap, 32-bit offsets) with a power-of-two unit count: each
+            // thread keeps one unit column and walks rows tid / nunit, + 256 / nunit, ...
+            const bool inrange = rbase >= 0 && rbase + nrows <= nsamps &&
+                                 static_cast<int64_t>(nrows) * row_bytes < (int64_t(1) << 31);
+            if (inrange && (nunit & (nunit - 1)) == 0) {
+                const int lg = __builtin_ctz(static_cast<unsigned>(nunit));
+                const int q = tid & (nunit - 1);
+                const int rstep = DD_THREADS >> lg;
+                const int rb32 = static_cast<int>(row_bytes);
+                const uint8_t* sbase = in + rbase * row_bytes + ub0 + UB * q;
+                for (int r0 = tid >> lg; r0 < nrows; r0 += DD_LD * rstep) {
+                    uint32_t v[DD_LD];
 #pragma unroll
-                    for (int jb = 0; jb < UB; ++jb) {
-                        const uint32_t byte = (v[u] >> (8 * jb)) & 0xffu;
+                    for (int u = 0; u < DD_LD; ++u) {
+                        const int r = r0 + u * rstep;
+                        v[u] = r < nrows ? load(sbase + r * rb32) : 0u;
+                    }
 #pragma unroll
-                        for (int k = 0; k < PER; ++k) {
-                            const int ch = col0 + jb * PER + k;
-                            if (ch >= 0 && ch < n) {
-                                const int shift = (NBITS == 8) ? 0 : (BIG ? (8 - (k + 1) * NBITS) : k * NBITS);
-                                tile[r * STRIDE + ch] = static_cast<uint8_t>((byte >> shift) & ((1u << NBITS) - 1u));
-                            }
-                        }
+                    for (int u = 0; u < DD_LD; ++u) {
+                        const int r = r0 + u * rstep;
+                        if (r < nrows) put(v[u], r, q);
+                    }
+                }
+            } else {
+                // Any unit count: element e is (row e / nunit, unit e % nunit), the division as a
+                // multiply-high (exact while e * nunit < 2^32: e < 768 * 65 here; magic 0 and the
+                // added e give e / 1).
+                const int nel = nrows * nunit;
+                const uint32_t magic = nunit > 1 ? static_cast<uint32_t>(0xffffffffu / static_cast<uint32_t>(nunit) + 1u) : 0u;
+                const uint32_t unit1 = nunit == 1 ? 1u : 0u;
+                for (int e0 = 0; e0 < nel; e0 += DD_LD * DD_THREADS) {
+                    uint32_t v[DD_LD];
+#pragma unroll
+                    for (int u = 0; u < DD_LD; ++u) {
+                        const int e = e0 + u * DD_THREADS + tid;
+                        const int r = static_cast<int>(__umulhi(static_cast<uint32_t>(e), magic) + unit1 * static_cast<uint32_t>(e));
+                        const int q = e - r * nunit;
+                        int64_t row = rbase + r;
+                        if (row < 0) row += nsamps;  // numba wraparound of a negative index
+                        v[u] = (e < nel && row >= 0 && row < nsamps) ? load(in + row * row_bytes + ub0 + UB * q) : 0u;
+                    }
+#pragma unroll
+                    for (int u = 0; u < DD_LD; ++u) {
+                        const int e = e0 + u * DD_THREADS + tid;
+                        const int r = static_cast<int>(__umulhi(static_cast<uint32_t>(e), magic) + unit1 * static_cast<uint32_t>(e));
+                        if (e < nel) put(v[u], r, e - r * nunit);
                     }
                 }
             }
         }
         __syncthreads();
-        // Channels in steps of 8 (delays read as two 16-byte broadcasts, 16 tile reads in
+        // Channels in steps of 8 (their tile offsets read as two 16-byte broadcasts, 16 tile reads in
         // flight) while the 8 share the current subband; single channels otherwise. The step
         // choice depends only on LDS values every thread sees, so it is uniform.
-        const int base0 = (tid - dmin) * STRIDE;  // + d * STRIDE lands in rows [tid, tid + spread]
+        const int base0 = tid * STRIDE;  // + soff[k]: row tid + rel, column k
         const int base1 = base0 + DD_THREADS * STRIDE;
         int k = 0;
         while (k < n) {
@@ -196,14 +215,14 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
                 }
             }
             if (step8) {
-                const int4 d0 = *reinterpret_cast<const int4*>(sdel + k);
-                const int4 d1 = *reinterpret_cast<const int4*>(sdel + k + 4);
+                const int4 d0 = *reinterpret_cast<const int4*>(soff + k);
+                const int4 d1 = *reinterpret_cast<const int4*>(soff + k + 4);
                 const int dd[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
                 uint32_t a[8], b[8];
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
-                    a[j] = tile[base0 + dd[j] * STRIDE + k + j];
-                    b[j] = tile[base1 + dd[j] * STRIDE + k + j];
+                    a[j] = tile[base0 + dd[j]];
+                    b[j] = tile[base1 + dd[j]];
                 }
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
@@ -219,9 +238,9 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
                         cur_sub = sub;
                     }
                 }
-                const int d = sdel[k];
-                acc0 += tile[base0 + d * STRIDE + k];
-                acc1 += tile[base1 + d * STRIDE + k];
+                const int d = soff[k];
+                acc0 += tile[base0 + d];
+                acc1 += tile[base1 + d];
                 ++k;
             }
         }
@@ -347,7 +366,9 @@ int launch_int(const uint8_t* in, int nbits, int big, const int32_t* delays, con
                                                         spp::ceil_div(nchans, int64_t(16))));
     }
     int64_t cpg = spp::ceil_div(nchans, groups);
-    cpg = spp::ceil_div(cpg, 8) * 8;
+    // Group starts on multiples of 64 channels keep full stages dword-aligned (a power-of-two
+    // unit count per stage); narrow groups on multiples of 8.
+    cpg = cpg > 64 ? spp::ceil_div(cpg, 64) * 64 : spp::ceil_div(cpg, 8) * 8;
     groups = spp::ceil_div(nchans, cpg);
     const int use_atomic = (MODE == 0 && groups > 1) ? 1 : 0;
     if (use_atomic) {
