@@ -73,6 +73,7 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
     __shared__ __align__(16) int32_t sdel[CCX];
     __shared__ __align__(16) int32_t ssub[CCX];
     __shared__ __align__(16) int32_t soff[CCX];  // tile offset of channel k's column at row rel
+    __shared__ int32_t sfast[CCX];  // MODE 1: band of channels k .. k+7 if they share one, else -2
     const int tid = threadIdx.x;
     const int64_t t0 = static_cast<int64_t>(blockIdx.x) * DD_T;
     const int64_t cb = static_cast<int64_t>(blockIdx.y) * chans_per_group;
@@ -109,7 +110,18 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
         }
         const int nrows = DD_T + (dmax - dmin);
         const int64_t rbase = t0 + dmin;
-        if (tid < n) soff[tid] = (sdel[tid] - dmin) * STRIDE + tid;  // read after the staging barrier
+        if (tid < n) {  // read after the staging barrier
+            soff[tid] = (sdel[tid] - dmin) * STRIDE + tid;
+            if constexpr (MODE == 1) {
+                int f = -2;
+                if (tid + 8 <= n) {
+                    f = ssub[tid];
+#pragma unroll
+                    for (int j = 1; j < 8; ++j) f = ssub[tid + j] == f ? f : -2;
+                }
+                sfast[tid] = f;
+            }
+        }
         {
             constexpr int PER = 8 / NBITS;  // samples per byte
             constexpr int UB = WIDE ? 4 : 1;  // bytes per unit
@@ -206,14 +218,7 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
         int k = 0;
         while (k < n) {
             bool step8 = k + 8 <= n;
-            if constexpr (MODE == 1) {
-                if (step8) {
-                    const int4 s0 = *reinterpret_cast<const int4*>(ssub + k);
-                    const int4 s1 = *reinterpret_cast<const int4*>(ssub + k + 4);
-                    step8 = s0.x == cur_sub && s0.y == cur_sub && s0.z == cur_sub && s0.w == cur_sub &&
-                            s1.x == cur_sub && s1.y == cur_sub && s1.z == cur_sub && s1.w == cur_sub;
-                }
-            }
+            if constexpr (MODE == 1) step8 = sfast[k] == cur_sub;  // -2 never equals a band
             if (step8) {
                 const int4 d0 = *reinterpret_cast<const int4*>(soff + k);
                 const int4 d1 = *reinterpret_cast<const int4*>(soff + k + 4);
@@ -236,6 +241,7 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
                     if (sub != cur_sub) {
                         flush_sub(cur_sub);
                         cur_sub = sub;
+                        continue;  // channel k again, now possibly as the start of a step of 8
                     }
                 }
                 const int d = soff[k];
