@@ -33,6 +33,43 @@ __global__ __launch_bounds__(RB_THREADS) void k_roll_rows(const float* __restric
     }
 }
 
+// The same roll with 16-byte stores: a thread writes RV float4s of row blockIdx.y (out rows
+// 16-byte aligned: out 16-byte aligned, out_cols % 4 == 0), reading the 4 sources of each as
+// plain loads (the source offset is arbitrary; a float4 that straddles the wrap takes its
+// elements one by one).
+constexpr int RV = 4;
+__global__ __launch_bounds__(RB_THREADS) void k_roll_rows4(const float* __restrict__ in, int64_t ncols,
+                                                           const int64_t* __restrict__ off,
+                                                           int64_t out_cols, float* __restrict__ out) {
+    const int64_t r = blockIdx.y;
+    const int64_t o = off[r];
+    const float* __restrict__ src = in + r * ncols;
+    float4* __restrict__ dst = reinterpret_cast<float4*>(out + r * out_cols);
+    const int64_t nv = out_cols / 4;
+    const int64_t v0 = static_cast<int64_t>(blockIdx.x) * RB_THREADS * RV + threadIdx.x;
+    float4 q[RV];
+#pragma unroll
+    for (int i = 0; i < RV; ++i) {
+        const int64_t v = v0 + static_cast<int64_t>(i) * RB_THREADS;
+        if (v >= nv) break;
+        int64_t j = 4 * v + o;
+        if (j >= ncols) j -= ncols;
+        if (j + 4 <= ncols) {
+            q[i] = make_float4(src[j], src[j + 1], src[j + 2], src[j + 3]);
+        } else {
+            float e[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) e[k] = src[j + k >= ncols ? j + k - ncols : j + k];
+            q[i] = make_float4(e[0], e[1], e[2], e[3]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < RV; ++i) {
+        const int64_t v = v0 + static_cast<int64_t>(i) * RB_THREADS;
+        if (v < nv) dst[v] = q[i];
+    }
+}
+
 // One thread per output sample of one DM row; channels in order.
 __global__ __launch_bounds__(RB_THREADS) void k_dmt(const float* __restrict__ in, int64_t nchans,
                                                     int64_t ncols, const int64_t* __restrict__ off,
@@ -75,6 +112,14 @@ extern "C" int spp_roll_rows(const float* in, int64_t nrows, int64_t ncols, cons
                         static_cast<long long>(r), static_cast<long long>(off_host[r]),
                         static_cast<long long>(out_cols));
         }
+    }
+    if (out_cols % 4 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0) {
+        const int64_t bx4 = spp::ceil_div(out_cols / 4, int64_t(RB_THREADS) * RV);
+        SPP_REQUIRE(bx4 <= 0x7fffffffLL, "too many columns");
+        hipLaunchKernelGGL(k_roll_rows4, dim3(static_cast<unsigned>(bx4), static_cast<unsigned>(nrows)),
+                           dim3(RB_THREADS), 0, spp::as_stream(stream), in, ncols, off, out_cols, out);
+        SPP_LAUNCHED("k_roll_rows4");
+        return SPP_OK;
     }
     const int64_t bx = std::min<int64_t>(spp::ceil_div(out_cols, int64_t(RB_THREADS)), 4096);
     hipLaunchKernelGGL(k_roll_rows, dim3(static_cast<unsigned>(bx), static_cast<unsigned>(nrows)),

@@ -1,8 +1,10 @@
-// Gulp pre-steps that share the dedispersion path's reads and unpack: per-channel sums
-// (bandpass), channel masking on the file's own bytes, and zero-DM removal.
+// Gulp pre-steps that share the dedispersion path's reads and unpack: per-sample channel sums
+// (collapse), per-channel sums (bandpass), channel masking on the file's own bytes, and
+// zero-DM removal.
 //
 // Gulp layout: file order [nsamps][nchans], packed nbits (1/2/4/8, bit order `big`) or
-// float32 (nbits == 32). All three are HBM-bound single passes over the gulp.
+// float32 (nbits == 32). Each is a single pass over the gulp (bandpass: plus a pass over its
+// per-chunk partial sums).
 #include "spp_common.hpp"
 #include "spp_unpack.hpp"
 
@@ -443,7 +445,7 @@ template <int NBITS, bool BIG, int UPT>
 __global__ __launch_bounds__(256) void k_zerodm_w(const uint8_t* __restrict__ in, int64_t nsamps,
                                                   int64_t nchans, int64_t row_bytes,
                                                   const float* __restrict__ w, const float* __restrict__ bpass,
-                                                  uint8_t* __restrict__ out) {
+                                                  uint8_t* __restrict__ out, int vec) {
     constexpr int PER = 8 / NBITS;
     constexpr int CPW = 4 * PER;  // channels per dword
     __shared__ uint32_t zpart[2][4];
@@ -451,6 +453,7 @@ __global__ __launch_bounds__(256) void k_zerodm_w(const uint8_t* __restrict__ in
     const int64_t nunit = row_bytes / 4;
     static_assert(UPT * CPW <= 32, "channels per thread");
     float wf[UPT][CPW], bf[UPT][CPW];
+    double wmax = 0.0, bmax = 0.0;
 #pragma unroll
     for (int i = 0; i < UPT; ++i) {
         const int64_t u = tid + 256 * i;
@@ -459,8 +462,15 @@ __global__ __launch_bounds__(256) void k_zerodm_w(const uint8_t* __restrict__ in
             const int64_t c = u * CPW + j;
             wf[i][j] = u < nunit ? w[c] : 0.f;
             bf[i][j] = u < nunit ? bpass[c] : 0.f;
+            wmax = fmax(wmax, fabs(static_cast<double>(wf[i][j])));
+            bmax = fmax(bmax, fabs(static_cast<double>(bf[i][j])));
         }
     }
+    // |y| <= 255 + z * |w| + |b| with z <= (2^nbits - 1) * nchans: when that stays below 2^30
+    // for all of this thread's channels, y converts to int32 without the clamp. (A NaN or
+    // infinite weight fails the test and keeps the clamped form.)
+    const double zmax = static_cast<double>((NBITS == 8 ? 255 : (1 << NBITS) - 1)) * static_cast<double>(nchans);
+    const bool direct = 255.0 + zmax * wmax + bmax < 1073741824.0;
     auto load_row = [&](int64_t t, uint32_t (&v)[UPT]) {
 #pragma unroll
         for (int i = 0; i < UPT; ++i) {
@@ -484,27 +494,40 @@ __global__ __launch_bounds__(256) void k_zerodm_w(const uint8_t* __restrict__ in
 #pragma unroll
         for (int i = 0; i < UPT; ++i) cur[i] = v[i];
         load_row(t + gridDim.x, v);  // next row in flight while this one is written
+        auto write_row = [&](auto clamp) {
 #pragma unroll
-        for (int i = 0; i < UPT; ++i) {
-            const int64_t u = tid + 256 * i;
-            if (u >= nunit) break;
-            uint32_t o[PER];
-            spp::expand_samples<NBITS, BIG>(cur[i], o);
-            uint32_t* dst = reinterpret_cast<uint32_t*>(out + t * nchans + u * CPW);
+            for (int i = 0; i < UPT; ++i) {
+                const int64_t u = tid + 256 * i;
+                if (u >= nunit) break;
+                uint32_t o[PER], res[PER];
+                spp::expand_samples<NBITS, BIG>(cur[i], o);
 #pragma unroll
-            for (int j = 0; j < PER; ++j) {
-                uint32_t res = 0;
+                for (int j = 0; j < PER; ++j) {
+                    res[j] = 0;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const double x = static_cast<double>((o[j] >> (8 * k)) & 0xffu);
-                    const double y = fma(-z, static_cast<double>(wf[i][4 * j + k]), x) +
-                                     static_cast<double>(bf[i][4 * j + k]);
-                    const double yc = fmin(fmax(y, -2147483648.0), 2147483647.0);
-                    res |= (static_cast<uint32_t>(static_cast<int32_t>(yc)) & 0xffu) << (8 * k);
+                    for (int k = 0; k < 4; ++k) {
+                        const double x = static_cast<double>((o[j] >> (8 * k)) & 0xffu);
+                        double y = fma(-z, static_cast<double>(wf[i][4 * j + k]), x) +
+                                   static_cast<double>(bf[i][4 * j + k]);
+                        if constexpr (decltype(clamp)::value) y = fmin(fmax(y, -2147483648.0), 2147483647.0);
+                        res[j] |= (static_cast<uint32_t>(static_cast<int32_t>(y)) & 0xffu) << (8 * k);
+                    }
                 }
-                dst[j] = res;
+                uint8_t* dst = out + t * nchans + u * CPW;  // CPW bytes, CPW-aligned when vec
+                if (vec && PER >= 4) {
+#pragma unroll
+                    for (int j = 0; j < PER; j += 4)
+                        reinterpret_cast<uint4*>(dst)[j / 4] = make_uint4(res[j], res[(j + 1) % PER], res[(j + 2) % PER], res[(j + 3) % PER]);
+                } else if (vec && PER == 2) {
+                    *reinterpret_cast<uint2*>(dst) = make_uint2(res[0], res[1 % PER]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < PER; ++j) reinterpret_cast<uint32_t*>(dst)[j] = res[j];
+                }
             }
-        }
+        };
+        if (direct) write_row(std::false_type{});
+        else write_row(std::true_type{});
     }
 }
 
@@ -687,8 +710,11 @@ extern "C" int spp_remove_zerodm(const void* in, int nbits, int big, int64_t nch
         const dim3 grid(static_cast<unsigned>(std::min<int64_t>(nsamps, 4 * spp::device_cu_count())));
         const auto* src = static_cast<const uint8_t*>(in);
         auto* dst = static_cast<uint8_t*>(out);
+        // vector stores of a thread's 4 * 8/nbits output bytes when they are that aligned (rows
+        // of nchans bytes keep it: 4 * 8/nbits divides nchans here)
+        const int vec = reinterpret_cast<uintptr_t>(out) % static_cast<uintptr_t>(std::min<int64_t>(16, cpw)) == 0 ? 1 : 0;
 #define SPP_ZDW(NB, BG, U) \
-    hipLaunchKernelGGL((k_zerodm_w<NB, BG, U>), grid, dim3(256), 0, st, src, nsamps, nchans, row_bytes, chanwts, bpass, dst)
+    hipLaunchKernelGGL((k_zerodm_w<NB, BG, U>), grid, dim3(256), 0, st, src, nsamps, nchans, row_bytes, chanwts, bpass, dst, vec)
 #define SPP_ZDW_U(NB, BG)                                              \
     do {                                                               \
         if constexpr (32 / NB >= 32) SPP_ZDW(NB, BG, 1);               \

@@ -235,6 +235,8 @@ def test_remove_zerodm_register_path(nbits, order, rng):
         fast = _zerodm(src, nbits, big, nchans, nsamps, bpass)
         slow = _zerodm(src, nbits, big, nchans, nsamps, bpass, out_offset=1)  # byte kernel
         np.testing.assert_array_equal(fast, slow, err_msg=f"{nchans}")
+        dword = _zerodm(src, nbits, big, nchans, nsamps, bpass, out_offset=4)  # dword stores only
+        np.testing.assert_array_equal(dword, slow, err_msg=f"{nchans} dword stores")
         # z is exact and y = x - z*w + b lands within a rounding of the unfused form
         ref = O.remove_zerodm(x.reshape(-1), nchans, bpass, np.uint8, fused=False)
         assert np.mean(fast == ref) > 0.99
@@ -255,4 +257,42 @@ def test_mask_bytes_unit_widths(row_bytes, offset, rng):
               _lib.stream_handle())
     got = dd[: data.size].cpu().numpy()
     exp = ((data.reshape(nrows, row_bytes) & keep) | setb).reshape(-1)
+    np.testing.assert_array_equal(got, exp)
+
+
+def test_remove_zerodm_large_weights_clamped(rng):
+    """Weights large enough that y leaves the int32 range: the clamped form (the low 8 bits of
+    the saturated value, as the per-row byte kernel computes) is kept for those threads."""
+    nchans, nsamps = 256, 20
+    x = rng.integers(0, 256, (nsamps, nchans), dtype=np.uint8)
+    bpass = rng.uniform(1.0e3, 5.0e4, nchans).astype(np.float32)
+    bpass[::7] = 3.0e9                        # b alone exceeds 2^31
+    _keep, src = _dev(x)
+    fast = _zerodm(src, 8, 0, nchans, nsamps, bpass)
+    slow = _zerodm(src, 8, 0, nchans, nsamps, bpass, out_offset=1)
+    np.testing.assert_array_equal(fast, slow)
+
+
+@pytest.mark.parametrize("ncols, out_cols, aligned", [(16384, 16384, True), (1000, 1000, True), (1001, 1001, True),
+                                                       (5000, 3000, True), (4096, 4096, False)])
+def test_roll_rows_vector_and_scalar(ncols, out_cols, aligned, rng):
+    """Rows rolled by arbitrary offsets (wrapping inside a 16-byte group included) through the
+    float4 path and the scalar path, against numpy."""
+    from sigpyproc3_amd import _lib
+
+    nrows = 37
+    x = rng.normal(size=(nrows, ncols)).astype(np.float32)
+    if out_cols == ncols:
+        off = rng.integers(0, ncols, nrows).astype(np.int64)
+        off[:4] = [0, 1, ncols - 1, ncols - 3]
+    else:
+        off = rng.integers(0, ncols - out_cols + 1, nrows).astype(np.int64)
+    _kx, xd = _dev(x)
+    od = _lib.to_device(off)
+    buf = _lib.zeros((nrows * out_cols + 4,), "float32")
+    out = buf if aligned else buf[1:]
+    _lib.call("spp_roll_rows", _lib.ptr(xd), nrows, ncols, _lib.ptr(off), _lib.ptr(od), out_cols, _lib.ptr(out),
+              _lib.stream_handle())
+    got = out[: nrows * out_cols].cpu().numpy().reshape(nrows, out_cols)
+    exp = np.stack([np.roll(x[r], -off[r])[:out_cols] for r in range(nrows)])
     np.testing.assert_array_equal(got, exp)
