@@ -296,3 +296,33 @@ def test_roll_rows_vector_and_scalar(ncols, out_cols, aligned, rng):
     got = out[: nrows * out_cols].cpu().numpy().reshape(nrows, out_cols)
     exp = np.stack([np.roll(x[r], -off[r])[:out_cols] for r in range(nrows)])
     np.testing.assert_array_equal(got, exp)
+
+
+@pytest.mark.parametrize("ncols, out_cols, ndm, kind", [(4096, 4096, 37, "smooth"), (3000, 3000, 20, "random"),
+                                                         (5000, 2000, 13, "smooth"), (300, 300, 5, "random")])
+def test_dmt_tiles_vs_sequential_sum(ncols, out_cols, ndm, kind, rng):
+    """The tiled DM-time transform (DM rows grouped by offset spread, channel windows in LDS)
+    against a float32 sum in channel order: smooth offsets (groups of up to 16 rows, ragged
+    last group) and random offsets (one row per block), circular and valid-columns forms."""
+    from sigpyproc3_amd import _lib
+
+    nchans = 67
+    x = rng.normal(size=(nchans, ncols)).astype(np.float32)
+    lim = ncols - out_cols + 1 if out_cols < ncols else ncols
+    if kind == "smooth":
+        base = np.linspace(0, min(lim - 1, 3 * ndm), ndm)[:, None] * np.linspace(0, 1, nchans)[None, :]
+        off = np.floor(base).astype(np.int64) % lim
+    else:
+        off = rng.integers(0, lim, (ndm, nchans)).astype(np.int64)
+    _kx, xd = _dev(x)
+    od = _lib.to_device(off.reshape(-1))
+    out = _lib.zeros((ndm * out_cols,), "float32")
+    offh = np.ascontiguousarray(off.reshape(-1))
+    _lib.call("spp_dmt", _lib.ptr(xd), nchans, ncols, _lib.ptr(offh), _lib.ptr(od), ndm, out_cols, _lib.ptr(out),
+              _lib.stream_handle())
+    got = out.cpu().numpy().reshape(ndm, out_cols)
+    t = np.arange(out_cols)
+    exp = np.zeros((ndm, out_cols), dtype=np.float32)
+    for c in range(nchans):
+        exp = (exp + x[c][(t[None, :] + off[:, c : c + 1]) % ncols]).astype(np.float32)
+    np.testing.assert_array_equal(got, exp)
