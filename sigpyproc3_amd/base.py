@@ -62,8 +62,9 @@ class _GulpPipeline:
     """Packed gulps into HBM with the file read of the next gulp overlapping the current one.
 
     Two pinned host buffers and two device buffers alternate. A reader thread runs
-    ``read_plan_packed_into`` (file reads release the GIL) and reads gulp k + 1 while gulp k
-    is copied to the device and its kernels run. A host buffer is handed back to the reader
+    ``read_plan_packed_into`` (file reads release the GIL; each gulp is read in parallel
+    pieces by the shared read pool) and reads gulp k + 1 while gulp k is copied to the device
+    and its kernels run. A host buffer is handed back to the reader
     once its copy has completed; a device buffer is rewritten only by a copy enqueued on the
     same stream after the kernels that read it.
     """
@@ -81,11 +82,14 @@ class _GulpPipeline:
         self.items: queue.Queue = queue.Queue()  # at most 2 entries: one per acquired buffer
         self._stop = False
         bufs = [h.numpy() for h in self.host]
+        from sigpyproc3_amd.io.fileio import read_pool
+
+        pool = read_pool()
 
         def produce() -> None:
             try:
                 gen = reader.read_plan_packed_into(bufs, gulp=gulp, start=start, nsamps=nsamps,
-                                                   skipback=skipback)
+                                                   skipback=skipback, pool=pool)
                 k = 0
                 while True:
                     self.free[k % 2].acquire()

@@ -80,6 +80,20 @@ class FileBase:
             self.ifile_cur = None
 
 
+_READ_POOL = None
+
+
+def read_pool():
+    """Threads shared by the parallel packed reads (at most 8)."""
+    global _READ_POOL
+    if _READ_POOL is None:
+        from concurrent.futures import ThreadPoolExecutor
+
+        _READ_POOL = ThreadPoolExecutor(max_workers=max(1, min(8, len(os.sched_getaffinity(0)))),
+                                        thread_name_prefix="spp-read")
+    return _READ_POOL
+
+
 class FileReader(FileBase):
     """Read a multi-file SIGPROC data stream (io/fileio.py:149-343)."""
 
@@ -136,6 +150,37 @@ class FileReader(FileBase):
     def creadinto_packed(self, read_buffer) -> int:
         """Fill ``read_buffer`` with packed stream bytes (no unpack)."""
         return self._readinto(read_buffer)
+
+    def creadinto_packed_parallel(self, read_buffer, pool, chunk: int = 1 << 23) -> int:
+        """``creadinto_packed`` with the request split into ``chunk``-byte pieces that the
+        threads of ``pool`` read at explicit offsets (``os.preadv`` releases the GIL, so page
+        cache copies run in parallel); the stream position then moves past the request. A
+        request that does not lie inside the current file (a file boundary, the end of the
+        stream) takes the sequential path."""
+        if self.ifile_cur is None:
+            msg = "No file is open for reading"
+            raise OSError(msg)
+        view = memoryview(read_buffer).cast("B")
+        total = len(view)
+        fd = self.file_obj.fileno()
+        pos = self.file_obj.tell()
+        if pool is None or total < 2 * chunk or pos + total > os.fstat(fd).st_size:
+            return self._readinto(read_buffer)
+
+        def piece(off: int) -> int:
+            end = min(off + chunk, total)
+            got = off
+            while got < end:
+                n = os.preadv(fd, [view[got:end]], pos + got)
+                if n <= 0:
+                    msg = f"short read at byte {pos + got}"
+                    raise OSError(msg)
+                got += n
+            return end - off
+
+        done = sum(pool.map(piece, range(0, total, chunk)))
+        self.file_obj.seek(pos + done)
+        return done
 
     def cread(self, nunits: int) -> np.ndarray:
         """Read ``nunits`` samples (unpacked for 1/2/4-bit data)."""

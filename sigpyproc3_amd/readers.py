@@ -113,10 +113,11 @@ class FilReader(Filterbank):
             yield nsamps_r, ii, raw
 
     def read_plan_packed_into(self, buffers, gulp: int = 16384, start: int = 0,
-                              nsamps: int | None = None, skipback: int = 0):
+                              nsamps: int | None = None, skipback: int = 0, pool=None):
         """The ``read_plan_packed`` schedule, reading gulp k into ``buffers[k % len(buffers)]``
         (each at least ``gulp * samp_stride`` bytes). Yields ``(nsamps_read, ii, k % n, nbytes)``;
-        the caller must be done with a buffer before asking for the gulp that reuses it."""
+        the caller must be done with a buffer before asking for the gulp that reuses it. With a
+        thread ``pool``, each gulp is read by its threads in parallel pieces."""
         if nsamps is None:
             nsamps = self.header.nsamples - start
         gulp, blocks = plan_blocks(nsamps, gulp, skipback)
@@ -129,14 +130,38 @@ class FilReader(Filterbank):
         nchans = self.header.nchans
         for k, (ii, nrows, skip) in enumerate(blocks):
             i = k % len(views)
-            nbytes = self._file.creadinto_packed(views[i][:need])
             expected = int(nrows * nchans * self.chan_stride)
+            # A full gulp buffer per read, as the reference's read_plan does (so a short last
+            # gulp that is not at the end of the stream fails the same byte-count check).
+            if pool is not None:
+                nbytes = self._file.creadinto_packed_parallel(views[i][:need], pool)
+            else:
+                nbytes = self._file.creadinto_packed(views[i][:need])
             if nbytes != expected:
                 msg = f"Unexpected number of bytes read from file {nbytes} (actual) != {expected} (expected)"
                 raise ValueError(msg)
             if skip != 0:
                 self._file.seek(int(skip * nchans * self.chan_stride), whence=1)
             yield nrows, ii, i, expected
+
+    def read_packed_to_device(self, host, dev, start: int, nsamps: int, pool, piece: int = 1 << 26) -> int:
+        """Packed bytes of samples ``[start, start + nsamps)`` into the pinned ``host`` tensor and
+        on to the device tensor ``dev``, ``piece`` bytes at a time: the asynchronous copy of a
+        piece (on the current stream) runs while the pool reads the next. Returns the bytes
+        read; a short read stops early."""
+        nbytes = int(nsamps) * self.samp_stride
+        buf = host.numpy()
+        self._file.seek(start * self.samp_stride)
+        done = 0
+        while done < nbytes:
+            n = min(piece, nbytes - done)
+            got = self._file.creadinto_packed_parallel(buf[done : done + n], pool)
+            if got > 0:
+                dev[done : done + got].copy_(host[done : done + got], non_blocking=True)
+            done += got
+            if got < n:
+                break
+        return done
 
     def _gulps(self, gulp, start, nsamps, skipback, *, allocator, unpack: bool):
         if nsamps is None:

@@ -148,6 +148,15 @@ def load_packed_stream(fil, start: int, nsamps: int):
     t = _lib.torch()
     nbytes = nsamps * fil.samp_stride
     host = t.empty(nbytes, dtype=t.uint8, pin_memory=True)
+    if hasattr(fil, "read_packed_to_device"):  # pool-read pieces, each copied while the next is read
+        from sigpyproc3_amd.io.fileio import read_pool
+
+        dev = _lib.empty((nbytes,), "uint8")
+        got = fil.read_packed_to_device(host, dev, start, nsamps, read_pool())
+        if got != nbytes:
+            msg = f"short read: {got} != {nbytes}"
+            raise ValueError(msg)
+        return dev
     buf = host.numpy()
     got = 0
     for _nr, _ii, raw in fil.read_plan_packed(gulp=nsamps, start=start, nsamps=nsamps,

@@ -145,6 +145,33 @@ def test_read_plan_packed_bytes(filfiles):
         pos += nr - (100 if nr == 1000 else 0)
 
 
+def test_parallel_packed_reads_match_sequential(filfiles):
+    """creadinto_packed_parallel (pieces read by a thread pool at explicit offsets) returns the
+    bytes of the sequential read and leaves the stream at the same position; a request that
+    crosses into the next file of a multi-file stream, or past its end, takes the sequential
+    path. Small pieces so that the fixtures split into many."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    pool = ThreadPoolExecutor(4)
+    for key in ("2bit", "8bit"):
+        seq, par = FilReader(filfiles[key]), FilReader(filfiles[key])
+        total = seq.header.nsamples * seq.samp_stride
+        cases = [(0, 100_000), (1234, 333_333), (total - 50_000, 50_000), (total - 40_000, 80_000)]
+        if key == "8bit":  # two files of 4096 samples: a read across the boundary
+            cases.append((4096 * seq.samp_stride - 10_000, 100_000))
+        for start, n in cases:
+            a = np.zeros(n, np.uint8)
+            b = np.zeros(n, np.uint8)
+            seq._file.seek(start)
+            par._file.seek(start)
+            na = seq._file.creadinto_packed(a)
+            nb = par._file.creadinto_packed_parallel(b, pool, chunk=4096)
+            assert na == nb
+            np.testing.assert_array_equal(a[:na], b[:nb])
+            assert seq._file.cur_data_pos_stream == par._file.cur_data_pos_stream
+    pool.shutdown()
+
+
 def test_read_plan_truncated_file(filfiles, tmp_path):
     # tests/test_readers.py:80-88: a short read raises ValueError
     h, raw = O.read_sigproc(filfiles["8bit"][0])
