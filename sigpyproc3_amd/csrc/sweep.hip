@@ -39,6 +39,8 @@
 
 namespace {
 
+typedef __attribute__((address_space(3))) void lds_void_t;  // an LDS address for the LDS-load builtin
+
 constexpr int SW_S = 256;                // interleave stride = words per wave row
 constexpr int SW_DT = 16;                // DM rows per tile
 constexpr int SW_LW = 512;               // LDS words per staged channel (2 KiB: a multiple of the
@@ -311,7 +313,6 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
     // each wave reads its own rows' bytes with scalar loads one stage ahead. Rows are
     // zero-padded past nchans (by more than two stages of at most 32 channels), so the last
     // stage's missing channels read their zero-filled windows at offset 0.
-    uint4 pf[CPW][NLD];
     int32_t bw_cur[CPW], bw_nxt[CPW];
     auto load_bw = [&](int st, int32_t* bw) {
 #pragma unroll
@@ -321,7 +322,12 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
             bw[i] = (st < nst && cl < CC && c < nchans) ? tbase[c] : 0;
         }
     };
-    auto prefetch = [&](int st) {
+    // Stage st's windows go straight from G into LDS buffer buf (buffer loads with the LDS
+    // flag: lane l's 16 bytes land at the wave-uniform destination + 16 l, the layout the
+    // reads below expect), with no register round trip. The buffer was last read in stage
+    // st - 1, before the barrier that ended it; the loads are waited for (vmcnt) before the
+    // barrier that ends stage st.
+    auto prefetch = [&](int st, int buf) {
         const int64_t c0 = static_cast<int64_t>(st) * CC;
 #pragma unroll
         for (int i = 0; i < CPW; ++i) {
@@ -337,25 +343,16 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
             const uint32_t off = static_cast<uint32_t>(t0 + bw_cur[i]) * 4u + 16u * lane;
 #pragma unroll
             for (int r = 0; r < NLD; ++r) {
-                if (4 * lane + 256 * r < stage_words) {
-                    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 1024u * r, 0, 0);
-                    pf[i][r] = make_uint4(v[0], v[1], v[2], v[3]);
-                }
+                if (4 * lane + 256 * r < stage_words)
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                        rs, (lds_void_t*)(&lds[buf * BUFW + cl * CHW + 256 * r]), 16,
+                        off + 1024u * r, 0, 0, 0);
             }
         }
     };
-    auto commit = [&](int buf) {
-#pragma unroll
-        for (int i = 0; i < CPW; ++i) {
-            const int cl = wave + WAVES * i;
-            if (cl >= CC) break;
-#pragma unroll
-            for (int r = 0; r < NLD; ++r) {
-                const int m = 4 * lane + 256 * r;
-                if (m < stage_words) *reinterpret_cast<uint4*>(&lds[buf * BUFW + cl * CHW + m]) = pf[i][r];
-            }
-        }
-    };
+    // vmcnt(0): the stage's LDS-bound loads have landed (expcnt and lgkmcnt left at their
+    // maxima, i.e. not waited for).
+    auto wait_loads = [&]() { __builtin_amdgcn_s_waitcnt(0x0f70); };
 
     const char* __restrict__ Lb = reinterpret_cast<const char*>(lds) + 4 * lane;
     const uint32_t* __restrict__ rrow[K];
@@ -369,8 +366,8 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
 
     load_bw(0, bw_cur);
     load_bw(1, bw_nxt);
-    prefetch(0);
-    commit(0);
+    prefetch(0, 0);
+    wait_loads();
     __syncthreads();
     int nb = 0, nw = 0;
     for (int st = 0; st < nst; ++st) {
@@ -379,7 +376,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
 #pragma unroll
             for (int i = 0; i < CPW; ++i) bw_cur[i] = bw_nxt[i];
             load_bw(st + 2, bw_nxt);
-            prefetch(st + 1);
+            prefetch(st + 1, buf ^ 1);
         }
         // The next stage's offsets, waited for at the next barrier (one stage past the last
         // reads the zero padding).
@@ -484,7 +481,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
                 nw = 0;
             }
         }
-        if (st + 1 < nst) commit(buf ^ 1);
+        if (st + 1 < nst) wait_loads();
 #pragma unroll
         for (int k = 0; k < K; ++k)
 #pragma unroll
