@@ -13,9 +13,9 @@
 // output samples. Channels are processed in stages of CC: wave w stages channels
 // w, w + WAVES, ... of the next stage, reading the windows G[c][jb .. jb + 256 + spread]
 // (jb = t0 + min delay of the tile, rounded down to 4 words) with raw buffer loads whose
-// range is the channel's valid words (past them the hardware returns 0), into registers
-// while the current stage is summed, then into the other half of a double-buffered LDS
-// area (one barrier per stage). Each wave owns K DM rows; per channel and DM row a lane
+// range is the channel's valid words (past them the hardware returns 0). The loads carry the
+// LDS flag: they write the other half of a double-buffered LDS area directly while the
+// current stage is summed (one vmcnt wait and one barrier per stage). Each wave owns K DM rows; per channel and DM row a lane
 // reads 4 LDS words (lanes read consecutive words: bank-conflict free) at a byte offset
 // the planner's relative delays give (scalar loads, one stage's row at a time), and adds
 // them into SWAR accumulators:
