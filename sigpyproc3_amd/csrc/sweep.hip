@@ -624,7 +624,9 @@ extern "C" int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int
         i = j;
     }
     p->use32 = static_cast<int64_t>(nchans) * maxv > 65535;
-    p->variant = 1;  // 8 waves x 2 DM rows x 20 (nibble) or 16 (byte) channels; tunable per plan
+    // 8 waves x 2 DM rows x 20 channels per stage (nibble data) or x 12 (byte data: 48 KB of
+    // LDS, three blocks per CU); tunable per plan
+    p->variant = nbits <= 2 ? 1 : 0;
     p->gpitch = ((nsamps + 63) / 64) * 64;
     const int64_t ntiles = static_cast<int64_t>(p->tile_dm0.size());
     int64_t off = 0;
@@ -768,7 +770,7 @@ int launch_sweep(const spp_sweep_plan* p, const Extent& e, char* ws, float* plan
             case 1: return launch_sweep_v<NBITS, USE32, 2, 8, 16, 4>(p, e, ws, plane, ld, st);
             case 2: return launch_sweep_v<NBITS, USE32, 4, 4, 8, 3>(p, e, ws, plane, ld, st);
             case 3: return launch_sweep_v<NBITS, USE32, 2, 8, 8, 4>(p, e, ws, plane, ld, st);
-            default: return launch_sweep_v<NBITS, USE32, 4, 4, 16, 2>(p, e, ws, plane, ld, st);
+            default: return launch_sweep_v<NBITS, USE32, 2, 8, 12, 6>(p, e, ws, plane, ld, st);
         }
     }
 }
