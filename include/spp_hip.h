@@ -99,8 +99,9 @@ int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int64_t nchans
                           int64_t nsamps, int nbits, int big, spp_sweep_plan** plan);
 int spp_sweep_plan_destroy(spp_sweep_plan* plan);
 /* Block shapes of the stage-2 kernel (DM rows per wave x waves x channels per stage):
- * 1/2-bit data: 0 = 2x8x16, 1 = 2x8x20 (default), 2 = 2x8x12, 3 = 2x8x8;
- * 4/8-bit data: 0 = 2x8x12 (default), 1 = 2x8x16, 2 = 4x4x8, 3 = 2x8x8.
+ * 1/2-bit data: 0 = 2x8x16, 1 = 2x8x20, 2 = 2x8x8 (6 waves per SIMD), 3 = 2x8x12 (6 waves per
+ *               SIMD, default);
+ * 4/8-bit data: 0 = 2x8x12 (6 waves per SIMD, default), 1 = 2x8x16, 2 = 4x4x8, 3 = 2x8x8.
  * All give identical results; spp_sweep_plan_set_variant picks one for a plan (the
  * autotuner times them). */
 int spp_sweep_variant_count(void);

@@ -624,9 +624,9 @@ extern "C" int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int
         i = j;
     }
     p->use32 = static_cast<int64_t>(nchans) * maxv > 65535;
-    // 8 waves x 2 DM rows x 20 channels per stage (nibble data) or x 12 (byte data: 48 KB of
-    // LDS, three blocks per CU); tunable per plan
-    p->variant = nbits <= 2 ? 1 : 0;
+    // 8 waves x 2 DM rows x 12 channels per stage at 6 waves per SIMD: 48 KB of LDS, three
+    // blocks per CU (nibble data: variant 3, byte data: variant 0); tunable per plan
+    p->variant = nbits <= 2 ? 3 : 0;
     p->gpitch = ((nsamps + 63) / 64) * 64;
     const int64_t ntiles = static_cast<int64_t>(p->tile_dm0.size());
     int64_t off = 0;
@@ -755,14 +755,16 @@ int launch_sweep_v(const spp_sweep_plan* p, const Extent& e, char* ws, float* pl
 // Block shapes: (DM rows per wave, waves, channels per stage, min waves per SIMD); all tile
 // SW_DT rows and take a multiple of 4 channels per stage (one packed offset word per 4).
 // Nibble data (1/2-bit): 20 channels per stage give 5-channel nibble groups for 2-bit data.
+// 6 waves per SIMD (<= 80 VGPRs, a few spilled around the lane widenings) with 12 channels per
+// stage (48 KB of LDS) fits three blocks per CU and is the fastest shape measured.
 template <int NBITS, bool USE32>
 int launch_sweep(const spp_sweep_plan* p, const Extent& e, char* ws, float* plane, int64_t ld,
                  hipStream_t st) {
     if constexpr (NBITS <= 2) {
         switch (p->variant) {
             case 1: return launch_sweep_v<NBITS, USE32, 2, 8, 20, 4>(p, e, ws, plane, ld, st);
-            case 2: return launch_sweep_v<NBITS, USE32, 2, 8, 12, 4>(p, e, ws, plane, ld, st);
-            case 3: return launch_sweep_v<NBITS, USE32, 2, 8, 8, 4>(p, e, ws, plane, ld, st);
+            case 2: return launch_sweep_v<NBITS, USE32, 2, 8, 8, 6>(p, e, ws, plane, ld, st);
+            case 3: return launch_sweep_v<NBITS, USE32, 2, 8, 12, 6>(p, e, ws, plane, ld, st);
             default: return launch_sweep_v<NBITS, USE32, 2, 8, 16, 4>(p, e, ws, plane, ld, st);
         }
     } else {
