@@ -755,8 +755,9 @@ int launch_sweep_v(const spp_sweep_plan* p, const Extent& e, char* ws, float* pl
 // Block shapes: (DM rows per wave, waves, channels per stage, min waves per SIMD); all tile
 // SW_DT rows and take a multiple of 4 channels per stage (one packed offset word per 4).
 // Nibble data (1/2-bit): 20 channels per stage give 5-channel nibble groups for 2-bit data.
-// 6 waves per SIMD (<= 80 VGPRs, a few spilled around the lane widenings) with 12 channels per
-// stage (48 KB of LDS) fits three blocks per CU and is the fastest shape measured.
+// 6 waves per SIMD (<= 80 VGPRs: two accumulators live in scratch, touched only at the byte ->
+// 16-bit widenings, the rest of the spills sit in the epilogue) with 12 channels per stage
+// (48 KB of LDS) fits three blocks per CU and is the fastest shape measured.
 template <int NBITS, bool USE32>
 int launch_sweep(const spp_sweep_plan* p, const Extent& e, char* ws, float* plane, int64_t ld,
                  hipStream_t st) {
