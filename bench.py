@@ -250,9 +250,10 @@ def bench_mf(args) -> int:
 
 def bench_single(args) -> int:
     """The one-DM path at 8-bit, 1024 ch, 65 536 samples, DM 300 (the gulp schedule of
-    Filterbank.dedisperse: gulp 16384, skipback = max delay), with the gulps already in HBM,
-    and the 2-bit unpack of a 256 MiB packed buffer. Both are HBM-bound: the roofline is
-    (bytes read + bytes written) per launch / kernel time against 8 TB/s."""
+    Filterbank.dedisperse: gulp 16384, skipback = max delay), with the gulps already in HBM;
+    one 2-bit 4096-channel gulp of 2^18 samples (the headline band) at DM 0, 100 and 1000;
+    and the 2-bit unpack of a 256 MiB packed buffer. Each is priced against HBM: (bytes read
+    + bytes written) per launch / kernel time against 8 TB/s."""
     import torch
 
     from sigpyproc3_amd import _lib
@@ -309,6 +310,37 @@ def bench_single(args) -> int:
     e1.record()
     torch.cuda.synchronize()
     t_graph = e0.elapsed_time(e1) / reps * 1e-3
+    # One 2-bit gulp of the 4096-channel band: 2^18 samples (256 MiB packed), one launch per DM.
+    from sigpyproc3_amd.synth import cfg_sweep_2bit
+
+    c2 = cfg_sweep_2bit()
+    c2.nsamps = 1 << 18
+    c2.pulses = []
+    packed2 = generate_packed(c2, device=_lib.device())
+    wide = {}
+    for dm in (0.0, 100.0, 1000.0):
+        d2 = c2.delays([dm])[0].astype(np.int32)
+        md2 = int(d2.max())
+        n_out = c2.nsamps - md2
+        d2_dev = _lib.to_device(d2)
+        out2 = _lib.empty((n_out,), "float32")
+
+        def gulp2(d2_dev=d2_dev, md2=md2, out2=out2):
+            _lib.call("spp_dedisperse", _lib.ptr(packed2), 2, 1, c2.nchans, c2.nsamps, _lib.ptr(d2_dev),
+                      md2, _lib.ptr(out2), 0, stream)
+
+        gulp2()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(20):
+            gulp2()
+        e1.record()
+        torch.cuda.synchronize()
+        t2 = e0.elapsed_time(e1) / 20 * 1e-3
+        b2 = c2.nsamps * c2.nchans // 4 + 4 * n_out
+        wide[f"dm{int(dm)}"] = {"us": t2 * 1e6, "max_delay": md2, "hbm_bytes": b2, "hbm_gbs": b2 / t2 / 1e9,
+                                "hbm_frac": b2 / t2 / 8e12, "channel_adds_per_s": n_out * c2.nchans / t2}
+    del packed2
     # unpack: 2-bit, 256 MiB packed -> 1 GiB
     nb = 256 << 20
     src = torch.randint(0, 256, (nb,), dtype=torch.uint8, device=_lib.device())
@@ -322,7 +354,7 @@ def bench_single(args) -> int:
     torch.cuda.synchronize()
     t_unpack = e0.elapsed_time(e1) / 10 * 1e-3
     result = {
-        "metric": "one-DM dedispersion (gulp path) and 2-bit unpack: HBM-bound kernels",
+        "metric": "one-DM dedispersion (gulp path) and 2-bit unpack, priced against HBM",
         "value": (cfg.nsamps - md) / t_pass,
         "unit": "output samples/s",
         "n_gpus": 1,
@@ -337,11 +369,13 @@ def bench_single(args) -> int:
                        "peak_sample": peak_t, "pulse_at": [20000, 20015],
                        "graph_us_per_pass": t_graph * 1e6, "graph_hbm_gbs": bytes_pass / t_graph / 1e9,
                        "graph_output_equal": graph_equal},
+        "dedisperse_2bit_4096ch_2p18": wide,
         "unpack_2bit": {"ms": t_unpack * 1e3, "bytes": nb * 5, "hbm_gbs": nb * 5 / t_unpack / 1e9,
                         "hbm_frac": nb * 5 / t_unpack / 8e12},
     }
     log(f"single: {t_pass * 1e6:.1f} us per dedisperse pass ({result['dedisperse']['hbm_gbs']:.0f} GB/s), "
-        f"{t_graph * 1e6:.1f} us replayed as a graph (output equal: {graph_equal}), "
+        f"{t_graph * 1e6:.1f} us replayed as a graph (output equal: {graph_equal}); 2-bit 4096 ch gulp "
+        + ", ".join(f"{k} {v['us']:.0f} us" for k, v in wide.items()) + "; "
         f"unpack {result['unpack_2bit']['hbm_gbs']:.0f} GB/s")
     print(json.dumps(result), flush=True)
     return 0

@@ -11,7 +11,9 @@
 // across blocks and combining with float atomics is bit-exact and order-free.
 //
 // Work per output sample: nchans adds; HBM bytes per gulp: nsamps * nchans * nbits/8
-// read + 4 * nout written. One DM per launch is HBM-bound.
+// read + 4 * nout written. For 1/2-bit data that is under a byte per add, and the kernel is
+// bound by instruction issue (a byte write per sample staged, a byte read and an add per
+// sample summed), not by HBM.
 #include "spp_common.hpp"
 #include "spp_unpack.hpp"
 
@@ -63,7 +65,7 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
     const uint8_t* __restrict__ in, const int32_t* __restrict__ delays,
     const int32_t* __restrict__ chan_to_sub, float* __restrict__ out, int64_t nchans, int64_t nsamps,
     int64_t nout, int64_t out_start, int64_t row_bytes, int64_t chans_per_group, int use_atomic,
-    int64_t nsubs, int64_t out_size) {
+    int64_t nsubs, int64_t out_size, int64_t ngroups, int64_t nblocks) {
     // CCX channels per stage at most; rows of CCX + 4 bytes (an odd number of dwords, so
     // the per-sample column reads are bank-conflict free). Narrow channel groups use the
     // small tile and fit more blocks per CU.
@@ -74,9 +76,15 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
     __shared__ __align__(16) int32_t ssub[CCX];
     __shared__ __align__(16) int32_t soff[CCX];  // tile offset of channel k's column at row rel
     __shared__ int32_t sfast[CCX];  // MODE 1: band of channels k .. k+7 if they share one, else -2
+    // XCD-aware order (blocks go to the 8 XCDs round-robin): consecutive logical blocks run on
+    // one XCD and are the channel groups of one time tile, so the groups that share a packed
+    // row's cache lines read them from that XCD's L2 at about the same time.
+    const int64_t per_xcd = gridDim.x / 8;
+    const int64_t logical = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (logical >= nblocks) return;
     const int tid = threadIdx.x;
-    const int64_t t0 = static_cast<int64_t>(blockIdx.x) * DD_T;
-    const int64_t cb = static_cast<int64_t>(blockIdx.y) * chans_per_group;
+    const int64_t t0 = (logical / ngroups) * DD_T;
+    const int64_t cb = (logical % ngroups) * chans_per_group;
     const int64_t ce = min(nchans, cb + chans_per_group);
     const int64_t ta = t0 + tid;
     const int64_t tb = t0 + DD_THREADS + tid;
@@ -362,14 +370,17 @@ int launch_int(const uint8_t* in, int nbits, int big, const int32_t* delays, con
     const int64_t row_bytes = nchans * nbits / 8;
     const int64_t bx = spp::ceil_div(nout, DD_T);
     SPP_REQUIRE(bx <= 0x7fffffffLL, "gulp too long");
-    // Split channels across blocks until the grid fills the device (integer sums are
-    // exact, so the atomic combination is order-free).
+    // Split channels across blocks until the grid fills the device, and into groups of at most
+    // one stage (DD_CC channels), so each block stages its rows once and the groups of a time
+    // tile share the packed rows through L2 (integer sums are exact, so the atomic
+    // combination is order-free).
     const int64_t cus = spp::device_cu_count();
     int64_t groups = 1;
     const int64_t maxval = (nbits == 8) ? 255 : ((1 << nbits) - 1);
     if (nchans * maxval < (int64_t(1) << 24)) {
         groups = std::max<int64_t>(1, std::min<int64_t>(spp::ceil_div(8 * cus, bx),
                                                         spp::ceil_div(nchans, int64_t(16))));
+        groups = std::max(groups, spp::ceil_div(nchans, int64_t(DD_CC)));
     }
     int64_t cpg = spp::ceil_div(nchans, groups);
     // Group starts on multiples of 64 channels keep full stages dword-aligned (a power-of-two
@@ -381,12 +392,15 @@ int launch_int(const uint8_t* in, int nbits, int big, const int32_t* delays, con
         int rc = fill_f32(out + out_start, nout, st);
         if (rc) return rc;
     }
-    dim3 grid(static_cast<unsigned>(bx), static_cast<unsigned>(groups));
+    const int64_t nblocks = bx * groups;
+    SPP_REQUIRE(nblocks <= 0x7ffffff0LL, "gulp too long");
+    dim3 grid(static_cast<unsigned>(spp::ceil_div(nblocks, int64_t(8)) * 8));
     const bool wide = row_bytes % 4 == 0 && reinterpret_cast<uintptr_t>(in) % 4 == 0;
     const bool narrow = cpg <= 16;
 #define SPP_GULP_ONE(NB, BG, W, CX)                                                                 \
     hipLaunchKernelGGL((k_gulp_int<NB, BG, MODE, W, CX>), grid, dim3(DD_THREADS), 0, st, in, delays, cts, \
-                       out, nchans, nsamps, nout, out_start, row_bytes, cpg, use_atomic, nsubs, out_size)
+                       out, nchans, nsamps, nout, out_start, row_bytes, cpg, use_atomic, nsubs, out_size, groups, \
+                       nblocks)
 #define SPP_GULP_LAUNCH(NB, BG)                  \
     if (wide && narrow)                          \
         SPP_GULP_ONE(NB, BG, true, 16);          \
