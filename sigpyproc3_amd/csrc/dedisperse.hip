@@ -71,7 +71,7 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
     // small tile and fit more blocks per CU.
     constexpr int STRIDE = CCX + 4;
     static_assert((STRIDE / 4) % 2 == 1 && STRIDE % 4 == 0, "odd dword row stride");
-    __shared__ uint8_t tile[DD_ROWS * STRIDE];
+    __shared__ __align__(16) uint8_t tile[DD_ROWS * STRIDE];
     __shared__ __align__(16) int32_t sdel[CCX];
     __shared__ __align__(16) int32_t ssub[CCX];
     __shared__ __align__(16) int32_t soff[CCX];  // tile offset of channel k's column at row rel
@@ -370,17 +370,18 @@ int launch_int(const uint8_t* in, int nbits, int big, const int32_t* delays, con
     const int64_t row_bytes = nchans * nbits / 8;
     const int64_t bx = spp::ceil_div(nout, DD_T);
     SPP_REQUIRE(bx <= 0x7fffffffLL, "gulp too long");
-    // Split channels across blocks until the grid fills the device, and into groups of at most
-    // one stage (DD_CC channels), so each block stages its rows once and the groups of a time
-    // tile share the packed rows through L2 (integer sums are exact, so the atomic
-    // combination is order-free).
+    // Split channels across blocks until the grid fills the device (integer sums are exact, so
+    // the atomic combination is order-free). Dedisperse also splits them into groups of at most
+    // one stage (DD_CC channels): each block stages its rows once and the groups of a time tile
+    // share the packed rows through L2. Subband keeps wider groups: its band partial sums are
+    // float atomics nsubs floats apart, and more groups multiply them.
     const int64_t cus = spp::device_cu_count();
     int64_t groups = 1;
     const int64_t maxval = (nbits == 8) ? 255 : ((1 << nbits) - 1);
     if (nchans * maxval < (int64_t(1) << 24)) {
         groups = std::max<int64_t>(1, std::min<int64_t>(spp::ceil_div(8 * cus, bx),
                                                         spp::ceil_div(nchans, int64_t(16))));
-        groups = std::max(groups, spp::ceil_div(nchans, int64_t(DD_CC)));
+        if (MODE == 0) groups = std::max(groups, spp::ceil_div(nchans, int64_t(DD_CC)));
     }
     int64_t cpg = spp::ceil_div(nchans, groups);
     // Group starts on multiples of 64 channels keep full stages dword-aligned (a power-of-two

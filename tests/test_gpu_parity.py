@@ -725,6 +725,47 @@ def test_packed_dedisperse_staging_paths(nbits, nchans, nsamps, misalign, rng):
     np.testing.assert_array_equal(out.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("nbits", [2, 8])
+@pytest.mark.parametrize("pattern", ["dm0", "runs5", "runs4_off_grid"])
+def test_packed_gulp_shared_delay_runs(nbits, pattern, rng):
+    """Channels in runs of one delay. At DM 0 every 4-channel dword of a tile row is summed by
+    one read and one byte sum; runs of 5 mix those with per-channel reads, and runs of 4 that
+    start at channel 2 never line up with the dword grid. Dedisperse, and subband with
+    6-channel bands (band changes off the 4-channel grid)."""
+    from sigpyproc3_amd import _lib
+
+    nchans, nsamps = 1024, 40000
+    order = O.DEFAULT_BITORDER.get(nbits, "big")
+    samples = rng.integers(0, 1 << nbits, nsamps * nchans, dtype=np.uint8)
+    packed = samples if nbits == 8 else O.pack(samples, nbits, order)
+    ch = np.arange(nchans)
+    if pattern == "dm0":
+        delays = np.zeros(nchans, np.int32)
+    elif pattern == "runs5":
+        delays = ((ch // 5) * 2).astype(np.int32)
+    else:
+        delays = (((ch + 2) // 4) * 3).astype(np.int32)
+    md = int(delays.max())
+    big = 1 if order == "big" else 0
+    dev = _lib.to_device(packed)
+    d_dev = _lib.to_device(delays)
+    out = _lib.zeros((nsamps - md,), "float32")
+    _lib.call("spp_dedisperse", _lib.ptr(dev), nbits, big, nchans, nsamps, _lib.ptr(d_dev), md,
+              _lib.ptr(out), 0, _lib.stream_handle())
+    ref = np.zeros(nsamps - md, np.float32)
+    c_oracle.dedisperse(samples, ref, delays, nchans, nsamps, 0)
+    np.testing.assert_array_equal(out.cpu().numpy(), ref)
+    cts = (ch // 6).astype(np.int32)
+    nsubs = int(cts.max()) + 1
+    sub = _lib.zeros(((nsamps - md) * nsubs,), "float32")
+    cts_dev = _lib.to_device(cts)
+    _lib.call("spp_subband", _lib.ptr(dev), nbits, big, nchans, nsamps, _lib.ptr(d_dev), _lib.ptr(cts_dev),
+              md, nsubs, _lib.ptr(sub), int(sub.numel()), _lib.stream_handle())
+    ref_sub = np.zeros((nsamps - md) * nsubs, np.float32)
+    c_oracle.subband(samples, ref_sub, delays, cts, md, nchans, nsubs, nsamps)
+    np.testing.assert_array_equal(sub.cpu().numpy(), ref_sub)
+
+
 def test_running_normalise_long_float_rows_and_wide_windows(rng):
     """Rows spanning several normalisation blocks with non-integer data (the sliding window
     sums round), and windows longer than the row (the mirrored padding covers the window)."""
