@@ -26,7 +26,7 @@ constexpr int DD_THREADS = 256;
 constexpr int DD_T = 512;       // output samples per block (2 per thread)
 constexpr int DD_CC = 64;       // channels per stage at most
 constexpr int DD_ROWS = 768;    // staged rows per stage: spread <= DD_ROWS - DD_T
-constexpr int DD_LD = 8;        // staging loads in flight per thread
+constexpr int DD_LD = 16;       // staging loads in flight per thread
 constexpr int DDF_CC = 16;      // float32 path: channels per stage
 constexpr int DDF_ROWS = 768;
 constexpr int DDF_STRIDE = 17;  // floats per staged row
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
     const uint8_t* __restrict__ in, const int32_t* __restrict__ delays,
     const int32_t* __restrict__ chan_to_sub, float* __restrict__ out, int64_t nchans, int64_t nsamps,
     int64_t nout, int64_t out_start, int64_t row_bytes, int64_t chans_per_group, int use_atomic,
-    int64_t nsubs, int64_t out_size, int64_t ngroups, int64_t nblocks) {
+    int64_t nsubs, int64_t out_size, int64_t ngroups, int64_t nblocks, int64_t ilv, int64_t nchunk) {
     // CCX channels per stage at most; rows of CCX + 4 bytes (an odd number of dwords, so
     // the per-sample column reads are bank-conflict free). Narrow channel groups use the
     // small tile and fit more blocks per CU.
@@ -84,8 +84,11 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
     if (logical >= nblocks) return;
     const int tid = threadIdx.x;
     const int64_t t0 = (logical / ngroups) * DD_T;
-    const int64_t cb = (logical % ngroups) * chans_per_group;
-    const int64_t ce = min(nchans, cb + chans_per_group);
+    // The block's channels: nchunk chunks of chans_per_group, chunk m starting at
+    // ((gh * nchunk + m) * ilv + gl) * chans_per_group for group g = gh * ilv + gl. With ilv > 1
+    // the ilv groups of a time tile that read one packed row line run side by side.
+    const int64_t grp = logical % ngroups;
+    const int64_t gl = grp % ilv, gh = grp / ilv;
     const int64_t ta = t0 + tid;
     const int64_t tb = t0 + DD_THREADS + tid;
     uint32_t acc0 = 0, acc1 = 0;
@@ -101,164 +104,169 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
         acc1 = 0;
     };
 
-    for (int64_t c = cb; c < ce;) {
-        const int ncand = static_cast<int>(min<int64_t>(CCX, ce - c));
-        __syncthreads();
-        if (tid < ncand) {
-            sdel[tid] = delays[c + tid];
-            if constexpr (MODE == 1) ssub[tid] = chan_to_sub[c + tid];
-        }
-        __syncthreads();
-        int dmin, dmax;
-        int n = stage_extent(sdel, ncand, DD_ROWS - DD_T, dmin, dmax);
-        if (n < ncand && n >= 4) {
-            // Keep stage starts on multiples of 4 channels (dword-aligned tile columns for the
-            // word unpack below); a prefix of a stage fits the rows too. dmin/dmax stay bounds.
-            n &= ~3;
-        }
-        const int nrows = DD_T + (dmax - dmin);
-        const int64_t rbase = t0 + dmin;
-        if (tid < n) {  // read after the staging barrier
-            soff[tid] = (sdel[tid] - dmin) * STRIDE + tid;
-            if constexpr (MODE == 1) {
-                int f = -2;
-                if (tid + 8 <= n) {
-                    f = ssub[tid];
-#pragma unroll
-                    for (int j = 1; j < 8; ++j) f = ssub[tid + j] == f ? f : -2;
-                }
-                sfast[tid] = f;
+    for (int64_t m = 0; m < nchunk; ++m) {
+        const int64_t cb = ((gh * nchunk + m) * ilv + gl) * chans_per_group;
+        if (cb >= nchans) break;
+        const int64_t ce = min(nchans, cb + chans_per_group);
+        for (int64_t c = cb; c < ce;) {
+            const int ncand = static_cast<int>(min<int64_t>(CCX, ce - c));
+            __syncthreads();
+            if (tid < ncand) {
+                sdel[tid] = delays[c + tid];
+                if constexpr (MODE == 1) ssub[tid] = chan_to_sub[c + tid];
             }
-        }
-        {
-            constexpr int PER = 8 / NBITS;  // samples per byte
-            constexpr int UB = WIDE ? 4 : 1;  // bytes per unit
-            const int64_t bb0 = c * NBITS / 8, bb1 = ((c + n) * NBITS + 7) / 8;
-            const int64_t ub0 = bb0 / UB * UB;  // first unit's byte
-            const int nunit = static_cast<int>((bb1 - ub0 + UB - 1) / UB);
-            const int col_base = static_cast<int>(ub0 * PER - c);  // tile column of unit 0's first sample
-            // One staged unit (row r of the stage, unit q): interior dwords expand to 4 * PER
-            // tile bytes with dword stores; edge units go byte by byte.
-            auto put = [&](uint32_t v, int r, int q) {
-                const int col0 = col_base + UB * PER * q;  // tile column of the unit's first sample
-                if constexpr (WIDE) {
-                    if (col0 >= 0 && col0 + 4 * PER <= n && (col0 & 3) == 0) {
-                        uint32_t o[PER];
-                        spp::expand_samples<NBITS, BIG>(v, o);
-                        uint32_t* dst = reinterpret_cast<uint32_t*>(tile + r * STRIDE + col0);
-#pragma unroll
-                        for (int j = 0; j < PER; ++j) dst[j] = o[j];
-                        return;
+            __syncthreads();
+            int dmin, dmax;
+            int n = stage_extent(sdel, ncand, DD_ROWS - DD_T, dmin, dmax);
+            if (n < ncand && n >= 4) {
+                // Keep stage starts on multiples of 4 channels (dword-aligned tile columns for the
+                // word unpack below); a prefix of a stage fits the rows too. dmin/dmax stay bounds.
+                n &= ~3;
+            }
+            const int nrows = DD_T + (dmax - dmin);
+            const int64_t rbase = t0 + dmin;
+            if (tid < n) {  // read after the staging barrier
+                soff[tid] = (sdel[tid] - dmin) * STRIDE + tid;
+                if constexpr (MODE == 1) {
+                    int f = -2;
+                    if (tid + 8 <= n) {
+                        f = ssub[tid];
+    #pragma unroll
+                        for (int j = 1; j < 8; ++j) f = ssub[tid + j] == f ? f : -2;
                     }
+                    sfast[tid] = f;
                 }
-#pragma unroll
-                for (int jb = 0; jb < UB; ++jb) {
-                    const uint32_t byte = (v >> (8 * jb)) & 0xffu;
-#pragma unroll
-                    for (int k = 0; k < PER; ++k) {
-                        const int ch = col0 + jb * PER + k;
-                        if (ch >= 0 && ch < n) {
-                            const int shift = (NBITS == 8) ? 0 : (BIG ? (8 - (k + 1) * NBITS) : k * NBITS);
-                            tile[r * STRIDE + ch] = static_cast<uint8_t>((byte >> shift) & ((1u << NBITS) - 1u));
+            }
+            {
+                constexpr int PER = 8 / NBITS;  // samples per byte
+                constexpr int UB = WIDE ? 4 : 1;  // bytes per unit
+                const int64_t bb0 = c * NBITS / 8, bb1 = ((c + n) * NBITS + 7) / 8;
+                const int64_t ub0 = bb0 / UB * UB;  // first unit's byte
+                const int nunit = static_cast<int>((bb1 - ub0 + UB - 1) / UB);
+                const int col_base = static_cast<int>(ub0 * PER - c);  // tile column of unit 0's first sample
+                // One staged unit (row r of the stage, unit q): interior dwords expand to 4 * PER
+                // tile bytes with dword stores; edge units go byte by byte.
+                auto put = [&](uint32_t v, int r, int q) {
+                    const int col0 = col_base + UB * PER * q;  // tile column of the unit's first sample
+                    if constexpr (WIDE) {
+                        if (col0 >= 0 && col0 + 4 * PER <= n && (col0 & 3) == 0) {
+                            uint32_t o[PER];
+                            spp::expand_samples<NBITS, BIG>(v, o);
+                            uint32_t* dst = reinterpret_cast<uint32_t*>(tile + r * STRIDE + col0);
+    #pragma unroll
+                            for (int j = 0; j < PER; ++j) dst[j] = o[j];
+                            return;
+                        }
+                    }
+    #pragma unroll
+                    for (int jb = 0; jb < UB; ++jb) {
+                        const uint32_t byte = (v >> (8 * jb)) & 0xffu;
+    #pragma unroll
+                        for (int k = 0; k < PER; ++k) {
+                            const int ch = col0 + jb * PER + k;
+                            if (ch >= 0 && ch < n) {
+                                const int shift = (NBITS == 8) ? 0 : (BIG ? (8 - (k + 1) * NBITS) : k * NBITS);
+                                tile[r * STRIDE + ch] = static_cast<uint8_t>((byte >> shift) & ((1u << NBITS) - 1u));
+                            }
+                        }
+                    }
+                };
+                auto load = [&](const uint8_t* src) -> uint32_t {
+                    return WIDE ? *reinterpret_cast<const uint32_t*>(src) : static_cast<uint32_t>(*src);
+                };
+                // Rows inside the gulp (no wrap, 32-bit offsets) with a power-of-two unit count: each
+                // thread keeps one unit column and walks rows tid / nunit, + 256 / nunit, ...
+                const bool inrange = rbase >= 0 && rbase + nrows <= nsamps &&
+                                     static_cast<int64_t>(nrows) * row_bytes < (int64_t(1) << 31);
+                if (inrange && (nunit & (nunit - 1)) == 0) {
+                    const int lg = __builtin_ctz(static_cast<unsigned>(nunit));
+                    const int q = tid & (nunit - 1);
+                    const int rstep = DD_THREADS >> lg;
+                    const int rb32 = static_cast<int>(row_bytes);
+                    const uint8_t* sbase = in + rbase * row_bytes + ub0 + UB * q;
+                    for (int r0 = tid >> lg; r0 < nrows; r0 += DD_LD * rstep) {
+                        uint32_t v[DD_LD];
+    #pragma unroll
+                        for (int u = 0; u < DD_LD; ++u) {
+                            const int r = r0 + u * rstep;
+                            v[u] = r < nrows ? load(sbase + r * rb32) : 0u;
+                        }
+    #pragma unroll
+                        for (int u = 0; u < DD_LD; ++u) {
+                            const int r = r0 + u * rstep;
+                            if (r < nrows) put(v[u], r, q);
+                        }
+                    }
+                } else {
+                    // Any unit count: element e is (row e / nunit, unit e % nunit), the division as a
+                    // multiply-high (exact while e * nunit < 2^32: e < 768 * 65 here; magic 0 and the
+                    // added e give e / 1).
+                    const int nel = nrows * nunit;
+                    const uint32_t magic = nunit > 1 ? static_cast<uint32_t>(0xffffffffu / static_cast<uint32_t>(nunit) + 1u) : 0u;
+                    const uint32_t unit1 = nunit == 1 ? 1u : 0u;
+                    for (int e0 = 0; e0 < nel; e0 += DD_LD * DD_THREADS) {
+                        uint32_t v[DD_LD];
+    #pragma unroll
+                        for (int u = 0; u < DD_LD; ++u) {
+                            const int e = e0 + u * DD_THREADS + tid;
+                            const int r = static_cast<int>(__umulhi(static_cast<uint32_t>(e), magic) + unit1 * static_cast<uint32_t>(e));
+                            const int q = e - r * nunit;
+                            int64_t row = rbase + r;
+                            if (row < 0) row += nsamps;  // numba wraparound of a negative index
+                            v[u] = (e < nel && row >= 0 && row < nsamps) ? load(in + row * row_bytes + ub0 + UB * q) : 0u;
+                        }
+    #pragma unroll
+                        for (int u = 0; u < DD_LD; ++u) {
+                            const int e = e0 + u * DD_THREADS + tid;
+                            const int r = static_cast<int>(__umulhi(static_cast<uint32_t>(e), magic) + unit1 * static_cast<uint32_t>(e));
+                            if (e < nel) put(v[u], r, e - r * nunit);
                         }
                     }
                 }
-            };
-            auto load = [&](const uint8_t* src) -> uint32_t {
-                return WIDE ? *reinterpret_cast<const uint32_t*>(src) : static_cast<uint32_t>(*src);
-            };
-            // Rows inside the gulp (no wrap, 32-bit offsets) with a power-of-two unit count: each
-            // thread keeps one unit column and walks rows tid / nunit, + 256 / nunit, ...
-            const bool inrange = rbase >= 0 && rbase + nrows <= nsamps &&
-                                 static_cast<int64_t>(nrows) * row_bytes < (int64_t(1) << 31);
-            if (inrange && (nunit & (nunit - 1)) == 0) {
-                const int lg = __builtin_ctz(static_cast<unsigned>(nunit));
-                const int q = tid & (nunit - 1);
-                const int rstep = DD_THREADS >> lg;
-                const int rb32 = static_cast<int>(row_bytes);
-                const uint8_t* sbase = in + rbase * row_bytes + ub0 + UB * q;
-                for (int r0 = tid >> lg; r0 < nrows; r0 += DD_LD * rstep) {
-                    uint32_t v[DD_LD];
-#pragma unroll
-                    for (int u = 0; u < DD_LD; ++u) {
-                        const int r = r0 + u * rstep;
-                        v[u] = r < nrows ? load(sbase + r * rb32) : 0u;
+            }
+            __syncthreads();
+            // Channels in steps of 8 (their tile offsets read as two 16-byte broadcasts, 16 tile reads in
+            // flight) while the 8 share the current subband; single channels otherwise. The step
+            // choice depends only on LDS values every thread sees, so it is uniform.
+            const int base0 = tid * STRIDE;  // + soff[k]: row tid + rel, column k
+            const int base1 = base0 + DD_THREADS * STRIDE;
+            int k = 0;
+            while (k < n) {
+                bool step8 = k + 8 <= n;
+                if constexpr (MODE == 1) step8 = sfast[k] == cur_sub;  // -2 never equals a band
+                if (step8) {
+                    const int4 d0 = *reinterpret_cast<const int4*>(soff + k);
+                    const int4 d1 = *reinterpret_cast<const int4*>(soff + k + 4);
+                    const int dd[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+                    uint32_t a[8], b[8];
+    #pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        a[j] = tile[base0 + dd[j]];
+                        b[j] = tile[base1 + dd[j]];
                     }
-#pragma unroll
-                    for (int u = 0; u < DD_LD; ++u) {
-                        const int r = r0 + u * rstep;
-                        if (r < nrows) put(v[u], r, q);
+    #pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        acc0 += a[j];
+                        acc1 += b[j];
                     }
-                }
-            } else {
-                // Any unit count: element e is (row e / nunit, unit e % nunit), the division as a
-                // multiply-high (exact while e * nunit < 2^32: e < 768 * 65 here; magic 0 and the
-                // added e give e / 1).
-                const int nel = nrows * nunit;
-                const uint32_t magic = nunit > 1 ? static_cast<uint32_t>(0xffffffffu / static_cast<uint32_t>(nunit) + 1u) : 0u;
-                const uint32_t unit1 = nunit == 1 ? 1u : 0u;
-                for (int e0 = 0; e0 < nel; e0 += DD_LD * DD_THREADS) {
-                    uint32_t v[DD_LD];
-#pragma unroll
-                    for (int u = 0; u < DD_LD; ++u) {
-                        const int e = e0 + u * DD_THREADS + tid;
-                        const int r = static_cast<int>(__umulhi(static_cast<uint32_t>(e), magic) + unit1 * static_cast<uint32_t>(e));
-                        const int q = e - r * nunit;
-                        int64_t row = rbase + r;
-                        if (row < 0) row += nsamps;  // numba wraparound of a negative index
-                        v[u] = (e < nel && row >= 0 && row < nsamps) ? load(in + row * row_bytes + ub0 + UB * q) : 0u;
+                    k += 8;
+                } else {
+                    if constexpr (MODE == 1) {
+                        const int sub = ssub[k];
+                        if (sub != cur_sub) {
+                            flush_sub(cur_sub);
+                            cur_sub = sub;
+                            continue;  // channel k again, now possibly as the start of a step of 8
+                        }
                     }
-#pragma unroll
-                    for (int u = 0; u < DD_LD; ++u) {
-                        const int e = e0 + u * DD_THREADS + tid;
-                        const int r = static_cast<int>(__umulhi(static_cast<uint32_t>(e), magic) + unit1 * static_cast<uint32_t>(e));
-                        if (e < nel) put(v[u], r, e - r * nunit);
-                    }
+                    const int d = soff[k];
+                    acc0 += tile[base0 + d];
+                    acc1 += tile[base1 + d];
+                    ++k;
                 }
             }
+            c += n;
         }
-        __syncthreads();
-        // Channels in steps of 8 (their tile offsets read as two 16-byte broadcasts, 16 tile reads in
-        // flight) while the 8 share the current subband; single channels otherwise. The step
-        // choice depends only on LDS values every thread sees, so it is uniform.
-        const int base0 = tid * STRIDE;  // + soff[k]: row tid + rel, column k
-        const int base1 = base0 + DD_THREADS * STRIDE;
-        int k = 0;
-        while (k < n) {
-            bool step8 = k + 8 <= n;
-            if constexpr (MODE == 1) step8 = sfast[k] == cur_sub;  // -2 never equals a band
-            if (step8) {
-                const int4 d0 = *reinterpret_cast<const int4*>(soff + k);
-                const int4 d1 = *reinterpret_cast<const int4*>(soff + k + 4);
-                const int dd[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
-                uint32_t a[8], b[8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    a[j] = tile[base0 + dd[j]];
-                    b[j] = tile[base1 + dd[j]];
-                }
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    acc0 += a[j];
-                    acc1 += b[j];
-                }
-                k += 8;
-            } else {
-                if constexpr (MODE == 1) {
-                    const int sub = ssub[k];
-                    if (sub != cur_sub) {
-                        flush_sub(cur_sub);
-                        cur_sub = sub;
-                        continue;  // channel k again, now possibly as the start of a step of 8
-                    }
-                }
-                const int d = soff[k];
-                acc0 += tile[base0 + d];
-                acc1 += tile[base1 + d];
-                ++k;
-            }
-        }
-        c += n;
     }
     if constexpr (MODE == 1) {
         flush_sub(cur_sub);
@@ -371,23 +379,43 @@ int launch_int(const uint8_t* in, int nbits, int big, const int32_t* delays, con
     const int64_t bx = spp::ceil_div(nout, DD_T);
     SPP_REQUIRE(bx <= 0x7fffffffLL, "gulp too long");
     // Split channels across blocks until the grid fills the device (integer sums are exact, so
-    // the atomic combination is order-free). Dedisperse also splits them into groups of at most
-    // one stage (DD_CC channels): each block stages its rows once and the groups of a time tile
-    // share the packed rows through L2. Subband keeps wider groups: its band partial sums are
-    // float atomics nsubs floats apart, and more groups multiply them.
+    // the atomic combination is order-free).
     const int64_t cus = spp::device_cu_count();
     int64_t groups = 1;
     const int64_t maxval = (nbits == 8) ? 255 : ((1 << nbits) - 1);
-    if (nchans * maxval < (int64_t(1) << 24)) {
+    const bool exact = nchans * maxval < (int64_t(1) << 24);
+    if (exact) {
         groups = std::max<int64_t>(1, std::min<int64_t>(spp::ceil_div(8 * cus, bx),
                                                         spp::ceil_div(nchans, int64_t(16))));
-        if (MODE == 0) groups = std::max(groups, spp::ceil_div(nchans, int64_t(DD_CC)));
     }
     int64_t cpg = spp::ceil_div(nchans, groups);
-    // Group starts on multiples of 64 channels keep full stages dword-aligned (a power-of-two
-    // unit count per stage); narrow groups on multiples of 8.
-    cpg = cpg > 64 ? spp::ceil_div(cpg, 64) * 64 : spp::ceil_div(cpg, 8) * 8;
-    groups = spp::ceil_div(nchans, cpg);
+    int64_t ilv = 1, nchunk = 1;
+    // Dedisperse with wide groups uses groups of DD_CC channels (one stage each), numbered so
+    // that the groups of a time tile run side by side on one XCD and read each packed row line
+    // (128 bytes) from L2 once it is fetched. When the mean delay spread across one row line is
+    // at most DD_T samples, the groups that share a line read mostly the same rows; then a block
+    // takes nchunk chunks one line apart, and the ilv = line / DD_CC groups of a line run side
+    // by side: the same single fetch with nchunk times fewer float atomics per output. Subband
+    // keeps contiguous groups: its band partial sums are float atomics nsubs floats apart, and
+    // splitting bands across groups multiplies them.
+    const int64_t line_ch = 1024 / nbits;
+    const int64_t line_spread = (nsamps - nout) * line_ch / nchans;
+    if (MODE == 0 && exact && cpg >= DD_CC && line_ch >= DD_CC && nchans >= line_ch) {
+        cpg = DD_CC;
+        if (line_spread <= DD_T) {
+            ilv = line_ch / DD_CC;
+            const int64_t hgroups = std::max<int64_t>(1, spp::ceil_div(groups, ilv));
+            nchunk = spp::ceil_div(nchans, line_ch * hgroups);
+            groups = ilv * spp::ceil_div(nchans, line_ch * nchunk);
+        } else {
+            groups = spp::ceil_div(nchans, int64_t(DD_CC));
+        }
+    } else {
+        // Group starts on multiples of 64 channels keep full stages dword-aligned (a power-of-two
+        // unit count per stage); narrow groups on multiples of 8.
+        cpg = cpg > 64 ? spp::ceil_div(cpg, 64) * 64 : spp::ceil_div(cpg, 8) * 8;
+        groups = spp::ceil_div(nchans, cpg);
+    }
     const int use_atomic = (MODE == 0 && groups > 1) ? 1 : 0;
     if (use_atomic) {
         int rc = fill_f32(out + out_start, nout, st);
@@ -401,7 +429,7 @@ int launch_int(const uint8_t* in, int nbits, int big, const int32_t* delays, con
 #define SPP_GULP_ONE(NB, BG, W, CX)                                                                 \
     hipLaunchKernelGGL((k_gulp_int<NB, BG, MODE, W, CX>), grid, dim3(DD_THREADS), 0, st, in, delays, cts, \
                        out, nchans, nsamps, nout, out_start, row_bytes, cpg, use_atomic, nsubs, out_size, groups, \
-                       nblocks)
+                       nblocks, ilv, nchunk)
 #define SPP_GULP_LAUNCH(NB, BG)                  \
     if (wide && narrow)                          \
         SPP_GULP_ONE(NB, BG, true, 16);          \
