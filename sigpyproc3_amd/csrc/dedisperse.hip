@@ -394,22 +394,27 @@ int launch_int(const uint8_t* in, int nbits, int big, const int32_t* delays, con
     // that the groups of a time tile run side by side on one XCD and read each packed row line
     // (128 bytes) from L2 once it is fetched. When the mean delay spread across one row line is
     // at most DD_T samples, the groups that share a line read mostly the same rows; then a block
-    // takes nchunk chunks one line apart, and the ilv = line / DD_CC groups of a line run side
+    // takes nchunk chunks one line apart, and the ilv = line / chunk groups of a line run side
     // by side: the same single fetch with nchunk times fewer float atomics per output. Subband
-    // keeps contiguous groups: its band partial sums are float atomics nsubs floats apart, and
-    // splitting bands across groups multiplies them.
+    // otherwise keeps contiguous groups: its band partial sums are float atomics nsubs floats
+    // apart, and splitting bands across groups multiplies them.
+    // Subband takes the same interleave with chunks of whole bands (the mean band width rounded
+    // up to DD_CC channels), so a band is still summed by one block and flushed once.
     const int64_t line_ch = 1024 / nbits;
     const int64_t line_spread = (nsamps - nout) * line_ch / nchans;
-    if (MODE == 0 && exact && cpg >= DD_CC && line_ch >= DD_CC && nchans >= line_ch) {
+    const int64_t chunk = MODE == 0 ? DD_CC
+                                    : spp::ceil_div(spp::ceil_div(nchans, std::max<int64_t>(1, nsubs)),
+                                                    int64_t(DD_CC)) * DD_CC;
+    const bool wide_groups = exact && cpg >= DD_CC && nchans >= line_ch;
+    if (wide_groups && line_spread <= DD_T && line_ch % chunk == 0) {
+        cpg = chunk;
+        ilv = line_ch / chunk;
+        const int64_t hgroups = std::max<int64_t>(1, spp::ceil_div(groups, ilv));
+        nchunk = spp::ceil_div(nchans, line_ch * hgroups);
+        groups = ilv * spp::ceil_div(nchans, line_ch * nchunk);
+    } else if (MODE == 0 && wide_groups) {
         cpg = DD_CC;
-        if (line_spread <= DD_T) {
-            ilv = line_ch / DD_CC;
-            const int64_t hgroups = std::max<int64_t>(1, spp::ceil_div(groups, ilv));
-            nchunk = spp::ceil_div(nchans, line_ch * hgroups);
-            groups = ilv * spp::ceil_div(nchans, line_ch * nchunk);
-        } else {
-            groups = spp::ceil_div(nchans, int64_t(DD_CC));
-        }
+        groups = spp::ceil_div(nchans, int64_t(DD_CC));
     } else {
         // Group starts on multiples of 64 channels keep full stages dword-aligned (a power-of-two
         // unit count per stage); narrow groups on multiples of 8.

@@ -791,6 +791,17 @@ def test_packed_dedisperse_long_gulps_line_groups(nbits, nchans, dm, rng):
     ref = np.zeros(nsamps - md, np.float32)
     c_oracle.dedisperse(samples, ref, delays, nchans, nsamps, 0)
     np.testing.assert_array_equal(out.cpu().numpy(), ref)
+    # Subband with 8 bands: chunks of whole 128-channel bands (125-channel bands for 1000
+    # channels straddle the chunks).
+    nsubs = 8
+    cts = (np.arange(nchans) // (nchans // nsubs)).astype(np.int32)
+    sub = _lib.zeros(((nsamps - md) * nsubs,), "float32")
+    cts_dev = _lib.to_device(cts)
+    _lib.call("spp_subband", _lib.ptr(dev), nbits, 1 if order == "big" else 0, nchans, nsamps, _lib.ptr(d_dev),
+              _lib.ptr(cts_dev), md, nsubs, _lib.ptr(sub), int(sub.numel()), _lib.stream_handle())
+    ref_sub = np.zeros((nsamps - md) * nsubs, np.float32)
+    c_oracle.subband(samples, ref_sub, delays, cts, md, nchans, nsubs, nsamps)
+    np.testing.assert_array_equal(sub.cpu().numpy(), ref_sub)
 
 
 def test_running_normalise_long_float_rows_and_wide_windows(rng):
