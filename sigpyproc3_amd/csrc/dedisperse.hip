@@ -26,6 +26,7 @@ constexpr int DD_THREADS = 256;
 constexpr int DD_T = 512;       // output samples per block (2 per thread)
 constexpr int DD_CC = 64;       // channels per stage at most
 constexpr int DD_ROWS = 768;    // staged rows per stage: spread <= DD_ROWS - DD_T
+constexpr int DD_ROWS_LOW = 576;  // the same for small delay spreads (four 64-channel tiles per CU)
 constexpr int DD_LD = 16;       // staging loads in flight per thread
 constexpr int DDF_CC = 16;      // float32 path: channels per stage
 constexpr int DDF_ROWS = 768;
@@ -60,7 +61,7 @@ __device__ __forceinline__ int stage_extent(const int32_t* sdel, int ncand, int 
 // (row, unit) elements, a unit being a dword (WIDE: rows and the stream start are 4-byte
 // aligned) or a byte, DD_LD of them in flight per thread, and unpack every sample of a unit
 // that belongs to the stage into its tile column.
-template <int NBITS, bool BIG, int MODE, bool WIDE, int CCX>
+template <int NBITS, bool BIG, int MODE, bool WIDE, int CCX, int ROWS>
 __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
     const uint8_t* __restrict__ in, const int32_t* __restrict__ delays,
     const int32_t* __restrict__ chan_to_sub, float* __restrict__ out, int64_t nchans, int64_t nsamps,
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
     // small tile and fit more blocks per CU.
     constexpr int STRIDE = CCX + 4;
     static_assert((STRIDE / 4) % 2 == 1 && STRIDE % 4 == 0, "odd dword row stride");
-    __shared__ __align__(16) uint8_t tile[DD_ROWS * STRIDE];
+    __shared__ __align__(16) uint8_t tile[ROWS * STRIDE];
     __shared__ __align__(16) int32_t sdel[CCX];
     __shared__ __align__(16) int32_t ssub[CCX];
     __shared__ __align__(16) int32_t soff[CCX];  // tile offset of channel k's column at row rel
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(DD_THREADS) void k_gulp_int(
             }
             __syncthreads();
             int dmin, dmax;
-            int n = stage_extent(sdel, ncand, DD_ROWS - DD_T, dmin, dmax);
+            int n = stage_extent(sdel, ncand, ROWS - DD_T, dmin, dmax);
             if (n < ncand && n >= 4) {
                 // Keep stage starts on multiples of 4 channels (dword-aligned tile columns for the
                 // word unpack below); a prefix of a stage fits the rows too. dmin/dmax stay bounds.
@@ -431,19 +432,26 @@ int launch_int(const uint8_t* in, int nbits, int big, const int32_t* delays, con
     dim3 grid(static_cast<unsigned>(spp::ceil_div(nblocks, int64_t(8)) * 8));
     const bool wide = row_bytes % 4 == 0 && reinterpret_cast<uintptr_t>(in) % 4 == 0;
     const bool narrow = cpg <= 16;
-#define SPP_GULP_ONE(NB, BG, W, CX)                                                                 \
-    hipLaunchKernelGGL((k_gulp_int<NB, BG, MODE, W, CX>), grid, dim3(DD_THREADS), 0, st, in, delays, cts, \
+    // 64-channel tiles of DD_ROWS_LOW rows (4 blocks per CU instead of 3) when twice the mean
+    // delay spread across DD_CC channels fits them; wider spreads split into more stages there.
+    const bool low = !narrow && (nsamps - nout) * 2 * DD_CC <= int64_t(DD_ROWS_LOW - DD_T) * nchans;
+#define SPP_GULP_ONE(NB, BG, W, CX, R)                                                              \
+    hipLaunchKernelGGL((k_gulp_int<NB, BG, MODE, W, CX, R>), grid, dim3(DD_THREADS), 0, st, in, delays, cts, \
                        out, nchans, nsamps, nout, out_start, row_bytes, cpg, use_atomic, nsubs, out_size, groups, \
                        nblocks, ilv, nchunk)
-#define SPP_GULP_LAUNCH(NB, BG)                  \
-    if (wide && narrow)                          \
-        SPP_GULP_ONE(NB, BG, true, 16);          \
-    else if (wide)                               \
-        SPP_GULP_ONE(NB, BG, true, DD_CC);       \
-    else if (narrow)                             \
-        SPP_GULP_ONE(NB, BG, false, 16);         \
-    else                                         \
-        SPP_GULP_ONE(NB, BG, false, DD_CC)
+#define SPP_GULP_LAUNCH(NB, BG)                            \
+    if (wide && narrow)                                    \
+        SPP_GULP_ONE(NB, BG, true, 16, DD_ROWS);           \
+    else if (wide && low)                                  \
+        SPP_GULP_ONE(NB, BG, true, DD_CC, DD_ROWS_LOW);    \
+    else if (wide)                                         \
+        SPP_GULP_ONE(NB, BG, true, DD_CC, DD_ROWS);        \
+    else if (narrow)                                       \
+        SPP_GULP_ONE(NB, BG, false, 16, DD_ROWS);          \
+    else if (low)                                          \
+        SPP_GULP_ONE(NB, BG, false, DD_CC, DD_ROWS_LOW);   \
+    else                                                   \
+        SPP_GULP_ONE(NB, BG, false, DD_CC, DD_ROWS)
     switch (nbits * 2 + (big ? 1 : 0)) {
         case 2: SPP_GULP_LAUNCH(1, false); break;
         case 3: SPP_GULP_LAUNCH(1, true); break;
