@@ -232,11 +232,19 @@ int spp_running_normalise(const float* x, int64_t ldx, int64_t nrows,
  *   zt = circular pad of z to Ng = 5-smooth good size >= n (kernels.py:998-1005),
  *   S = sum(zt); then the first maximum over [width][t] in template-major order
  *   (np.argmax over convs, filters.py:333-338).
+ * That closed form is the reference's result when Ng is even. When Ng is odd the
+ * reference's irfft has its default length Ng - 1 (kernels.py:904): the row's conv is
+ *   y = IDFT_{Ng-1}(Y), Y[k] = DFT_Ng(conv)[k] for 0 <= k <= (Ng-1)/2 (imaginary parts of
+ *   the first and last of those bins dropped), Y[Ng-1-k] = conj(Y[k]),
+ * computed in float64 through FFTs; a row of exactly Ng samples (Ng odd) cannot be filled
+ * by Ng - 1 samples, and is refused with SPP_EINVAL as the reference raises.
  * widths_host: int32 [nwidths] (1 <= w <= 1024, nwidths <= 32). Outputs per row (device):
  * snr (float32 conv value at the maximum), peak (t), widx (index into widths).
- * workspace: spp_boxcar_mf_workspace_size bytes. Synchronises the stream.
+ * workspace: spp_boxcar_mf_workspace_size bytes for the same row lengths and widths.
+ * Synchronises the stream.
  */
-int spp_boxcar_mf_workspace_size(int64_t nrows, int64_t max_len, int64_t* bytes);
+int spp_boxcar_mf_workspace_size(int64_t nrows, const int64_t* row_len_host,
+                                 const int32_t* widths_host, int32_t nwidths, int64_t* bytes);
 int spp_boxcar_mf(const float* z, int64_t ldz, int64_t nrows, const int64_t* row_len_host,
                   const int32_t* widths_host, int32_t nwidths, float* snr, int32_t* peak,
                   int32_t* widx, void* workspace, int64_t workspace_bytes, void* stream);
@@ -247,6 +255,8 @@ int spp_boxcar_mf(const float* z, int64_t ldz, int64_t nrows, const int64_t* row
  * sigma_k with mu_k = sum(h_k)/Ng and sigma_k = sqrt(sum(h_k^2) - sum(h_k)^2/Ng) (the
  * normalize_template of core/kernels.py:1008-1027 applied to the padded template;
  * sigma_k == 0 leaves the numerator unscaled). Computed in float64, stored float32.
+ * For an odd Ng the reference's default-length irfft (kernels.py:904) is reproduced as for
+ * spp_boxcar_mf, and n == Ng (odd) is refused with SPP_EINVAL.
  * temps_host: concatenated float32 template samples; temp_len_host / ref_bin_host: int32
  * [ntemps]. convs: float32 [ntemps][n]. workspace: spp_convolve_templates_workspace_size.
  */

@@ -26,8 +26,14 @@ filterbank_dedisperse      independence for non-negative delays; notebook KAT
 subband_kernel             nsub == 1 reduces to dedisperse; brute force
 sweep_plane                row i == filterbank_dedisperse(delays[i])
 good_size_real             scipy.fft.next_fast_len(n, real=True) (pocketfft)
-convolve_templates_fft     the closed form below to 1e-9 (fp64)
-boxcar_conv_closed         convolve_templates_fft(fp64); brute-force circular sum
+convolve_templates_fft     explicit DFT sums (no FFT) of the padded series and template
+                           with irfft_default_brute, at odd and even good sizes; raises
+                           for n == Ng odd (n = 243, 625, ...)
+irfft_default_brute        numpy irfft's default-length output on random spectra,
+                           including nonzero imaginary parts in the first/last bins
+circular_to_reference      convolve_templates_fft (fp64) at odd and even good sizes
+boxcar_conv_closed,        convolve_templates_fft (fp64); brute-force circular sums
+template_conv_closed       (boxcar_conv_circular / template_conv_circular)
 estimate_zscore            textbook median / IQR via np.percentile (stats.py:480-502)
 running_filter_mean        brute-force window mean over the symmetric-padded series
 running_normalise          PARITY UNPINNED by the reference: cfg "running mean/std"
@@ -521,7 +527,12 @@ def normalize_template(arr):
 
 
 def convolve_templates_fft(data, temps, ref_bins, dtype=np.float64):
-    """kernels.convolve_templates (kernels.py:861-906) with numpy FFTs in ``dtype``."""
+    """kernels.convolve_templates (kernels.py:861-906) with numpy FFTs in ``dtype``.
+
+    ``np.fft.irfft`` is called with its default length, as at kernels.py:904: for an odd
+    good size Ng that is Ng - 1 samples, and when n == Ng (odd) the row cannot be filled,
+    so this raises ValueError as the reference's slice assignment (kernels.py:905) does.
+    """
     data = np.asarray(data, dtype=dtype)
     n = data.size
     ng = good_size_real(n)
@@ -534,17 +545,42 @@ def convolve_templates_fft(data, temps, ref_bins, dtype=np.float64):
         tp = np.roll(tp, -rb)
         tp = np.roll(tp[::-1], 1)
         tn = normalize_template(tp)
-        conv = np.fft.irfft(data_fft * np.fft.rfft(tn), n=ng)
+        conv = np.fft.irfft(data_fft * np.fft.rfft(tn))
+        if conv.size < n:
+            msg = f"cannot assign slice from input of different size ({conv.size} < {n})"
+            raise ValueError(msg)
         convs[i] = conv[:n]
     return convs
 
 
-def template_conv_closed(z, h, ref_bin: int) -> np.ndarray:
-    """Closed form of one convolve_templates row in float64.
+def circular_to_reference(c_full, n: int) -> np.ndarray:
+    """The reference's ``conv[:n]`` from the Ng-circular convolution ``c_full`` (last axis
+    of length Ng, all Ng lags).
 
-    ``conv[t] = (sum_m h_m zt[(t - ref + m) mod Ng] - mu * sum(zt)) / sigma`` with
-    ``zt`` the circular pad of ``z`` to ``Ng``, ``mu = sum(h)/Ng`` and
-    ``sigma = sqrt(sum(h^2) - sum(h)^2/Ng)`` (the unit-power normalisation).
+    kernels.py:904 computes ``irfft(rfft(pad(z)) * rfft(t))`` = ``irfft(rfft(c))`` at
+    irfft's default length 2 * (Ng // 2): that is ``c`` itself for an even Ng, and for an odd
+    Ng the length-(Ng - 1) inverse of the first (Ng + 1) / 2 bins of ``rfft(c)`` (the
+    imaginary parts of the first and last of them dropped). Raises ValueError when those
+    Ng - 1 samples cannot fill n outputs (n == Ng odd), as kernels.py:905 does.
+    """
+    c_full = np.asarray(c_full, dtype=np.float64)
+    ng = c_full.shape[-1]
+    if ng % 2 == 0:
+        return c_full[..., :n]
+    y = np.fft.irfft(np.fft.rfft(c_full, axis=-1), axis=-1)
+    if y.shape[-1] < n:
+        msg = f"cannot assign slice from input of different size ({y.shape[-1]} < {n})"
+        raise ValueError(msg)
+    return y[..., :n]
+
+
+def template_conv_circular(z, h, ref_bin: int) -> np.ndarray:
+    """Ng-circular convolution of one template, float64, at all Ng lags.
+
+    ``c[t] = (sum_m h_m zt[(t - ref + m) mod Ng] - mu * sum(zt)) / sigma`` with ``zt`` the
+    circular pad of ``z`` to Ng (kernels.py:998-1005), ``mu = sum(h)/Ng`` and
+    ``sigma = sqrt(sum(h^2) - sum(h)^2/Ng)`` (normalize_template, kernels.py:1008-1027, of
+    the padded, rolled and reversed template, kernels.py:896-902).
     """
     z = np.asarray(z, dtype=np.float64)
     h = np.asarray(h, dtype=np.float64)
@@ -554,30 +590,59 @@ def template_conv_closed(z, h, ref_bin: int) -> np.ndarray:
     s = zt.sum()
     mu = h.sum() / ng
     var = (h * h).sum() - h.sum() ** 2 / ng
-    t = np.arange(n)
-    acc = np.zeros(n)
+    t = np.arange(ng)
+    acc = np.zeros(ng)
     for m in range(h.size):
         acc += h[m] * zt[(t - ref_bin + m) % ng]
     num = acc - mu * s
     return num / np.sqrt(var) if var > 0 else num
 
 
-def boxcar_conv_closed(z, widths) -> np.ndarray:
-    """F10 closed form for boxcar templates, float64, shape [len(widths), n]."""
+def template_conv_closed(z, h, ref_bin: int) -> np.ndarray:
+    """One convolve_templates row (kernels.py:861-906) in float64: the circular closed
+    form mapped to the reference's output length (``circular_to_reference``)."""
+    z = np.asarray(z)
+    return circular_to_reference(template_conv_circular(z, h, ref_bin), z.size)
+
+
+def boxcar_conv_circular(z, widths) -> np.ndarray:
+    """F10 closed form for boxcar templates (ones(w), ref 0), float64, at all Ng lags:
+    ``(sum_{j<w} zt[(t + j) mod Ng] - w S / Ng) / sqrt(w (1 - w / Ng))``."""
     z = np.asarray(z, dtype=np.float64)
     n = z.size
     ng = good_size_real(n)
     zt = z[np.arange(ng) % n]
     q = np.concatenate([[0.0], np.cumsum(zt)])
     s = q[-1]
-    t = np.arange(n)
-    out = np.empty((len(widths), n))
+    t = np.arange(ng)
+    out = np.empty((len(widths), ng))
     for i, w in enumerate(widths):
         w = int(w)
         end = t + w
         win = np.where(end <= ng, q[np.minimum(end, ng)] - q[t], (s - q[t]) + q[np.maximum(end - ng, 0)])
         out[i] = (win - w * s / ng) / math.sqrt(w * (1.0 - w / ng))
     return out
+
+
+def boxcar_conv_closed(z, widths) -> np.ndarray:
+    """Boxcar convolve_templates rows [len(widths), n] in float64 (the reference's output
+    for any n: ``circular_to_reference`` of the closed form)."""
+    z = np.asarray(z)
+    return circular_to_reference(boxcar_conv_circular(z, widths), z.size)
+
+
+def irfft_default_brute(spec_half) -> np.ndarray:
+    """numpy's irfft at its default length m = 2 (len(spec_half) - 1), written out as a
+    real inverse DFT: ``y_t = (Re X_0 + 2 sum_{0<k<m/2} Re(X_k e^{2 pi i k t/m})
+    + Re X_{m/2} (-1)^t) / m`` (the imaginary parts of X_0 and X_{m/2} do not enter)."""
+    x = np.asarray(spec_half, dtype=np.complex128)
+    m = 2 * (x.size - 1)
+    t = np.arange(m)
+    y = np.full(m, x[0].real)
+    for k in range(1, x.size - 1):
+        y += 2.0 * (x[k] * np.exp(2j * np.pi * k * t / m)).real
+    y += x[-1].real * np.where(t % 2 == 0, 1.0, -1.0)
+    return y / m
 
 
 def mf_argmax(convs):

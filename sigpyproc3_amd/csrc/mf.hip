@@ -19,7 +19,10 @@
 //   zt[i] = z[(i mod Ng) mod n], S = sum_{i<Ng} zt[i],
 //   conv_w[t] = (sum_{j<w} zt[t+j] - w*S/Ng) / sqrt(w*(1 - w/Ng)),  t < n,
 //   argmax over [width][t] in template-major order with the first maximum kept.
-#include "spp_common.hpp"
+// That is the reference's result when Ng is even. For an odd Ng the reference's irfft has
+// Ng - 1 output samples and the row goes through the FFT path of spp_fft.hpp instead; a
+// row of exactly Ng (odd) samples is refused, as the reference cannot fill it either.
+#include "spp_fft.hpp"
 
 #include <algorithm>
 #include <cfloat>
@@ -281,6 +284,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_circ_sum(const float* __restrict
     __shared__ double red[MF_THREADS / 64];
     const int64_t row = blockIdx.y;
     const MfRow r = rows[row];
+    if (r.ng & 1) return;  // odd good size: the FFT path (spp_fft.hpp) handles the row
     const float* __restrict__ zr = z + row * ldz;
     const int64_t per = (r.ng + MF_NSP - 1) / MF_NSP;
     const int64_t i0 = static_cast<int64_t>(blockIdx.x) * per, i1 = min(i0 + per, r.ng);
@@ -307,7 +311,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_boxcar_part(
     best.val = -FLT_MAX;
     best.w = 0x7fffffff;
     best.t = INT64_MAX;
-    if (a >= r.n) {
+    if (a >= r.n || (r.ng & 1)) {  // odd good size: partials come from k_conv_argmax_part
         if (tid == 0) part[row * chunk_pitch + blockIdx.x] = best;
         return;
     }
@@ -430,13 +434,7 @@ __global__ void k_boxcar_final(const Best* __restrict__ part, int64_t nrows, int
 
 // ---------------------------------------------------------------- template bank
 
-struct TempDesc {
-    int32_t len;
-    int32_t ref;
-    int64_t off;   // offset of the taps in the tap array
-    double mu;     // sum(h) / Ng
-    double inv;    // 1 / sigma (or 1 if sigma == 0)
-};
+using spp::TempDesc;
 
 __global__ __launch_bounds__(MF_THREADS) void k_template_conv(
     const float* __restrict__ z, int64_t n, int64_t ng, const double* __restrict__ S,
@@ -456,13 +454,15 @@ __global__ __launch_bounds__(MF_THREADS) void k_template_conv(
     }
     for (int m = tid; m < td.len; m += MF_THREADS) hs[m] = taps[td.off + m];
     __syncthreads();
-    const double mus = td.mu * S[0];
+    double mu, inv;
+    spp::temp_norm(td, ng, &mu, &inv);
+    const double mus = mu * S[0];
     for (int u = tid; u < CT_B; u += MF_THREADS) {
         const int64_t t = a + u;
         if (t >= n) break;
         double acc = 0.0;
         for (int m = 0; m < td.len; ++m) acc += static_cast<double>(hs[m]) * static_cast<double>(zs[u + m]);
-        convs[static_cast<int64_t>(blockIdx.y) * n + t] = static_cast<float>((acc - mus) * td.inv);
+        convs[static_cast<int64_t>(blockIdx.y) * n + t] = static_cast<float>((acc - mus) * inv);
     }
 }
 
@@ -475,18 +475,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_circ_sum1(const float* __restric
     if (threadIdx.x == 0) S[0] = s;
 }
 
-int64_t good_size_real(int64_t n) {
-    // Smallest 2^a 3^b 5^c >= n (pocketfft's good_size for real transforms).
-    if (n <= 6) return n;
-    int64_t best = 2 * n;
-    for (int64_t f5 = 1; f5 < best; f5 *= 5)
-        for (int64_t f35 = f5; f35 < best; f35 *= 3) {
-            int64_t x = f35;
-            while (x < n) x *= 2;
-            best = std::min(best, x);
-        }
-    return best;
-}
+using spp::good_size_real;
 
 struct NormLayout {
     int64_t off_rows = 0, bytes = 0;
@@ -550,25 +539,87 @@ extern "C" int spp_running_normalise(const float* x, int64_t ldx, int64_t nrows,
 
 namespace {
 struct MfLayout {
-    int64_t chunk_pitch = 0, off_rows = 0, off_S = 0, off_w = 0, off_part = 0, bytes = 0;
+    int64_t chunk_pitch = 0, off_rows = 0, off_S = 0, off_w = 0, off_part = 0;
+    // rows whose good size is odd (see spp_fft.hpp): descriptors, taps, convs, FFT space
+    int64_t odd_n = 0, off_td = 0, off_taps = 0, off_convs = 0, off_fft = 0, fft_bytes = 0;
+    int64_t bytes = 0;
 };
-MfLayout mf_layout(int64_t nrows, int64_t max_len) {
+MfLayout mf_layout(int64_t nrows, const int64_t* row_len, const int32_t* widths, int32_t nw) {
     MfLayout l;
+    int64_t max_len = 1, sumw = 0;
+    for (int64_t i = 0; i < nrows; ++i) {
+        max_len = std::max(max_len, row_len[i]);
+        const int64_t ng = good_size_real(row_len[i]);
+        if ((ng & 1) && ng > row_len[i]) {
+            l.odd_n = std::max(l.odd_n, row_len[i]);
+            l.fft_bytes = std::max(l.fft_bytes, spp::odd_conv_workspace(row_len[i], nw));
+        }
+    }
+    for (int i = 0; i < nw; ++i) sumw += widths[i];
     l.chunk_pitch = std::max<int64_t>(1, (max_len + MF_B - 1) / MF_B);
     int64_t off = 0;
     l.off_rows = off; off = align256(off + static_cast<int64_t>(sizeof(MfRow)) * nrows);
     l.off_S = off; off = align256(off + 8 * MF_NSP * nrows);
     l.off_w = off; off = align256(off + 4 * MF_MAXW);
     l.off_part = off; off = align256(off + static_cast<int64_t>(sizeof(Best)) * nrows * l.chunk_pitch);
+    if (l.odd_n > 0) {
+        l.off_td = off; off = align256(off + static_cast<int64_t>(sizeof(TempDesc)) * nw);
+        l.off_taps = off; off = align256(off + 4 * sumw);
+        l.off_convs = off; off = align256(off + 4 * l.odd_n * nw);
+        l.off_fft = off; off = align256(off + l.fft_bytes);
+    }
     l.bytes = off;
     return l;
 }
+
+// The first maximum over convs[w][t] (t in one MF_B chunk of a row of n samples) in
+// template-major order, as k_boxcar_part's partial for that chunk.
+__global__ __launch_bounds__(MF_THREADS) void k_conv_argmax_part(const float* __restrict__ convs,
+                                                                 int64_t ldc, int64_t n, int nw,
+                                                                 Best* __restrict__ part) {
+    __shared__ Best bred[MF_THREADS];
+    const int64_t a = static_cast<int64_t>(blockIdx.x) * MF_B;
+    const int tid = threadIdx.x;
+    Best best;
+    best.val = -FLT_MAX;
+    best.w = 0x7fffffff;
+    best.t = INT64_MAX;
+    for (int wi = 0; wi < nw; ++wi) {
+        for (int u = tid; u < MF_B; u += MF_THREADS) {
+            const int64_t t = a + u;
+            if (t >= n) break;
+            const float v = convs[static_cast<int64_t>(wi) * ldc + t];
+            if (better(v, wi, t, best)) {
+                best.val = v;
+                best.w = wi;
+                best.t = t;
+            }
+        }
+    }
+    bred[tid] = best;
+    __syncthreads();
+    for (int s = MF_THREADS / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            const Best o = bred[tid + s];
+            if (better(o.val, o.w, o.t, bred[tid])) bred[tid] = o;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) part[blockIdx.x] = bred[0];
+}
 }  // namespace
 
-extern "C" int spp_boxcar_mf_workspace_size(int64_t nrows, int64_t max_len, int64_t* bytes) {
+extern "C" int spp_boxcar_mf_workspace_size(int64_t nrows, const int64_t* row_len_host,
+                                            const int32_t* widths_host, int32_t nwidths, int64_t* bytes) {
     spp::clear_error();
-    SPP_REQUIRE(nrows > 0 && max_len > 0 && bytes != nullptr, "invalid argument");
-    *bytes = mf_layout(nrows, max_len).bytes;
+    SPP_REQUIRE(nrows > 0 && row_len_host != nullptr && widths_host != nullptr && bytes != nullptr,
+                "invalid argument");
+    SPP_REQUIRE(nwidths >= 1 && nwidths <= MF_MAXW, "nwidths must be in [1, %d]", MF_MAXW);
+    for (int64_t i = 0; i < nrows; ++i) SPP_REQUIRE(row_len_host[i] >= 1, "row lengths must be >= 1");
+    for (int i = 0; i < nwidths; ++i)
+        SPP_REQUIRE(widths_host[i] >= 1 && widths_host[i] <= MF_WMAX, "width %d outside [1, %d]",
+                    widths_host[i], MF_WMAX);
+    *bytes = mf_layout(nrows, row_len_host, widths_host, nwidths).bytes;
     return SPP_OK;
 }
 
@@ -588,7 +639,7 @@ extern "C" int spp_boxcar_mf(const float* z, int64_t ldz, int64_t nrows, const i
         wmax = std::max(wmax, widths_host[i]);
     }
     std::vector<MfRow> rows(nrows);
-    int64_t maxn = 0;
+    std::vector<int64_t> odd_rows;
     for (int64_t i = 0; i < nrows; ++i) {
         const int64_t n = row_len_host[i];
         SPP_REQUIRE(n >= 1 && n <= ldz, "row %lld: length %lld outside [1, ldz]",
@@ -597,9 +648,15 @@ extern "C" int spp_boxcar_mf(const float* z, int64_t ldz, int64_t nrows, const i
                     static_cast<long long>(i), wmax, static_cast<long long>(n));
         rows[i].n = n;
         rows[i].ng = good_size_real(n);
-        maxn = std::max(maxn, n);
+        // The reference's default-length irfft gives Ng - 1 samples for an odd Ng, too few
+        // to fill a row of Ng samples (core/kernels.py:904-905): it raises there.
+        SPP_REQUIRE(!((rows[i].ng & 1) && rows[i].ng == n),
+                    "row %lld: length %lld is an odd FFT good size; the reference's irfft returns "
+                    "%lld samples and cannot fill the row",
+                    static_cast<long long>(i), static_cast<long long>(n), static_cast<long long>(n - 1));
+        if (rows[i].ng & 1) odd_rows.push_back(i);
     }
-    const MfLayout l = mf_layout(nrows, maxn);
+    const MfLayout l = mf_layout(nrows, row_len_host, widths_host, nwidths);
     SPP_REQUIRE(workspace_bytes >= l.bytes, "workspace too small: %lld < %lld",
                 static_cast<long long>(workspace_bytes), static_cast<long long>(l.bytes));
     hipStream_t st = spp::as_stream(stream);
@@ -617,6 +674,35 @@ extern "C" int spp_boxcar_mf(const float* z, int64_t ldz, int64_t nrows, const i
                        dim3(static_cast<unsigned>(l.chunk_pitch), static_cast<unsigned>(nrows)),
                        dim3(MF_THREADS), 0, st, z, ldz, drows, S, dw, nwidths, wmax, l.chunk_pitch, part);
     SPP_LAUNCHED("k_boxcar_part");
+    // Rows with an odd good size: the template bank through the FFT path, then their chunk
+    // partials (k_boxcar_part left those rows' partials neutral).
+    std::vector<TempDesc> td(nwidths);
+    std::vector<float> taps;
+    if (!odd_rows.empty()) {
+        for (int i = 0; i < nwidths; ++i) {
+            const int32_t w = widths_host[i];
+            td[i].len = w;
+            td[i].ref = 0;
+            td[i].off = static_cast<int64_t>(taps.size());
+            td[i].sh = w;
+            td[i].sh2 = w;
+            taps.insert(taps.end(), w, 1.0f);
+        }
+        TempDesc* dtd = reinterpret_cast<TempDesc*>(ws + l.off_td);
+        float* dtaps = reinterpret_cast<float*>(ws + l.off_taps);
+        float* convs = reinterpret_cast<float*>(ws + l.off_convs);
+        SPP_HIP(hipMemcpyAsync(dtd, td.data(), sizeof(TempDesc) * nwidths, hipMemcpyHostToDevice, st));
+        SPP_HIP(hipMemcpyAsync(dtaps, taps.data(), 4 * taps.size(), hipMemcpyHostToDevice, st));
+        for (const int64_t row : odd_rows) {
+            const int64_t n = rows[row].n;
+            const int rc = spp::odd_convolve(z + row * ldz, n, dtaps, dtd, nwidths, convs, l.odd_n,
+                                             ws + l.off_fft, l.fft_bytes, st);
+            if (rc != SPP_OK) return rc;
+            hipLaunchKernelGGL(k_conv_argmax_part, dim3(static_cast<unsigned>(spp::ceil_div(n, MF_B))),
+                               dim3(MF_THREADS), 0, st, convs, l.odd_n, n, nwidths, part + row * l.chunk_pitch);
+            SPP_LAUNCHED("k_conv_argmax_part");
+        }
+    }
     hipLaunchKernelGGL(k_boxcar_final, dim3(static_cast<unsigned>(spp::ceil_div(nrows, 64))), dim3(64), 0,
                        st, part, nrows, l.chunk_pitch, drows, snr, peak, widx);
     SPP_LAUNCHED("k_boxcar_final");
@@ -624,12 +710,30 @@ extern "C" int spp_boxcar_mf(const float* z, int64_t ldz, int64_t nrows, const i
     return SPP_OK;
 }
 
+namespace {
+struct CtLayout {
+    int64_t off_td = 0, off_taps = 0, off_fft = 0, fft_bytes = 0, bytes = 0;
+};
+CtLayout ct_layout(int64_t n, int32_t ntemps, int64_t total_taps) {
+    CtLayout l;
+    int64_t off = align256(8);
+    l.off_td = off; off = align256(off + static_cast<int64_t>(sizeof(TempDesc)) * ntemps);
+    l.off_taps = off; off = align256(off + 4 * total_taps);
+    const int64_t ng = good_size_real(n);
+    if ((ng & 1) && ng > n) {
+        l.fft_bytes = spp::odd_conv_workspace(n, ntemps);
+        l.off_fft = off; off = align256(off + l.fft_bytes);
+    }
+    l.bytes = off;
+    return l;
+}
+}  // namespace
+
 extern "C" int spp_convolve_templates_workspace_size(int64_t n, int32_t ntemps, int64_t total_taps,
                                                      int64_t* bytes) {
     spp::clear_error();
     SPP_REQUIRE(n > 0 && ntemps > 0 && total_taps > 0 && bytes != nullptr, "invalid argument");
-    *bytes = align256(8) + align256(static_cast<int64_t>(sizeof(TempDesc)) * ntemps) +
-             align256(4 * total_taps);
+    *bytes = ct_layout(n, ntemps, total_taps).bytes;
     return SPP_OK;
 }
 
@@ -644,6 +748,10 @@ extern "C" int spp_convolve_templates(const float* data, int64_t n, const float*
     SPP_REQUIRE(n > 0, "n must be > 0");
     SPP_REQUIRE(ntemps > 0 && ntemps <= 65535, "ntemps must be in [1, 65535]");
     const int64_t ng = good_size_real(n);
+    SPP_REQUIRE(!((ng & 1) && ng == n),
+                "length %lld is an odd FFT good size: the reference's irfft returns %lld samples, "
+                "too few for the output (core/kernels.py:904-905)",
+                static_cast<long long>(n), static_cast<long long>(n - 1));
     std::vector<TempDesc> td(ntemps);
     int64_t off = 0;
     for (int i = 0; i < ntemps; ++i) {
@@ -657,30 +765,35 @@ extern "C" int spp_convolve_templates(const float* data, int64_t n, const float*
             s += h;
             s2 += h * h;
         }
-        const double var = s2 - s * s / static_cast<double>(ng);
         td[i].len = len;
         td[i].ref = ref_bin_host[i];
         td[i].off = off;
-        td[i].mu = s / static_cast<double>(ng);
-        td[i].inv = var > 0.0 ? 1.0 / std::sqrt(var) : 1.0;
+        td[i].sh = s;
+        td[i].sh2 = s2;
         off += len;
     }
-    int64_t need = 0;
-    spp_convolve_templates_workspace_size(n, ntemps, off, &need);
-    SPP_REQUIRE(workspace_bytes >= need, "workspace too small");
+    const CtLayout l = ct_layout(n, ntemps, off);
+    SPP_REQUIRE(workspace_bytes >= l.bytes, "workspace too small: %lld < %lld",
+                static_cast<long long>(workspace_bytes), static_cast<long long>(l.bytes));
     hipStream_t st = spp::as_stream(stream);
     char* ws = static_cast<char*>(workspace);
     double* S = reinterpret_cast<double*>(ws);
-    TempDesc* dtd = reinterpret_cast<TempDesc*>(ws + align256(8));
-    float* taps = reinterpret_cast<float*>(ws + align256(8) + align256(sizeof(TempDesc) * ntemps));
+    TempDesc* dtd = reinterpret_cast<TempDesc*>(ws + l.off_td);
+    float* taps = reinterpret_cast<float*>(ws + l.off_taps);
     SPP_HIP(hipMemcpyAsync(dtd, td.data(), sizeof(TempDesc) * ntemps, hipMemcpyHostToDevice, st));
     SPP_HIP(hipMemcpyAsync(taps, temps_host, 4 * off, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_circ_sum1, dim3(1), dim3(MF_THREADS), 0, st, data, n, ng, S);
-    SPP_LAUNCHED("k_circ_sum1");
-    hipLaunchKernelGGL(k_template_conv,
-                       dim3(static_cast<unsigned>(spp::ceil_div(n, CT_B)), static_cast<unsigned>(ntemps)),
-                       dim3(MF_THREADS), 0, st, data, n, ng, S, taps, dtd, convs);
-    SPP_LAUNCHED("k_template_conv");
+    if (l.fft_bytes > 0) {
+        // Odd good size: the reference's default-length irfft (spp_fft.hpp).
+        const int rc = spp::odd_convolve(data, n, taps, dtd, ntemps, convs, n, ws + l.off_fft, l.fft_bytes, st);
+        if (rc != SPP_OK) return rc;
+    } else {
+        hipLaunchKernelGGL(k_circ_sum1, dim3(1), dim3(MF_THREADS), 0, st, data, n, ng, S);
+        SPP_LAUNCHED("k_circ_sum1");
+        hipLaunchKernelGGL(k_template_conv,
+                           dim3(static_cast<unsigned>(spp::ceil_div(n, CT_B)), static_cast<unsigned>(ntemps)),
+                           dim3(MF_THREADS), 0, st, data, n, ng, S, taps, dtd, convs);
+        SPP_LAUNCHED("k_template_conv");
+    }
     SPP_HIP(hipStreamSynchronize(st));
     return SPP_OK;
 }

@@ -198,7 +198,8 @@ def boxcar_mf(z, row_len, widths=DEFAULT_WIDTHS, *, stream=None):
     rl = np.ascontiguousarray(row_len, dtype=np.int64)
     w = np.ascontiguousarray(widths, dtype=np.int32)
     need = np.zeros(1, dtype=np.int64)
-    _lib.call("spp_boxcar_mf_workspace_size", nrows, int(rl.max()), need.ctypes.data_as(_lib._P64))
+    _lib.call("spp_boxcar_mf_workspace_size", nrows, _lib.ptr(rl), _lib.ptr(w), int(w.size),
+              need.ctypes.data_as(_lib._P64))
     ws = _lib.empty((int(need[0]),), "uint8")
     snr = _lib.empty((nrows,), "float32")
     peak = _lib.empty((nrows,), "int32")

@@ -214,10 +214,82 @@ def test_box_widths():
     assert w[0] == 1 and (np.diff(w) > 0).all() and w[-1] <= 16
 
 
-@pytest.mark.parametrize("n", [1000, 4093, 30001])
+def test_irfft_default_brute_matches_numpy(rng):
+    # numpy's default irfft length is 2 * (m - 1); the imaginary parts of the first and last
+    # bins do not enter (they are nonzero here on purpose)
+    for m in (2, 3, 8, 63, 1563):
+        x = rng.normal(size=m) + 1j * rng.normal(size=m)
+        np.testing.assert_allclose(O.irfft_default_brute(x), np.fft.irfft(x), atol=1e-12, rtol=0)
+
+
+def _explicit_reference_convs(z, temps, ref_bins):
+    """kernels.convolve_templates from its definition with explicit DFT sums (no FFT)."""
+    z = np.asarray(z, dtype=np.float64)
+    n = z.size
+    ng = O.good_size_real(n)
+    zt = z[np.arange(ng) % n]
+    k = np.arange(ng // 2 + 1)
+    ph = np.exp(-2j * np.pi * np.outer(k, np.arange(ng)) / ng)
+    d = ph @ zt
+    out = []
+    for h, rb in zip(temps, ref_bins):
+        tp = np.zeros(ng)
+        tp[: len(h)] = h
+        tp = np.roll(np.roll(tp, -rb)[::-1], 1)
+        tp = tp - tp.mean()
+        norm = np.sqrt((tp**2).sum())
+        tn = tp / norm if norm != 0 else tp
+        y = O.irfft_default_brute(d * (ph @ tn))
+        if y.size < n:
+            raise ValueError("short")
+        out.append(y[:n])
+    return np.array(out)
+
+
+@pytest.mark.parametrize("n", [13, 50, 121, 1000, 3100])
+def test_convolve_templates_fft_vs_explicit_dft(n, rng):
+    # odd good sizes: 13 -> 15, 121 -> 125, 3100 -> 3125; even: 50, 1000
+    z = rng.normal(size=n)
+    x = np.arange(-4, 5)
+    temps = [np.ones(1), np.ones(4), np.exp(-0.5 * x**2 / 4.0)]
+    refs = [0, 0, 4]
+    ref = _explicit_reference_convs(z, temps, refs)
+    np.testing.assert_allclose(O.convolve_templates_fft(z, temps, refs), ref, atol=1e-10, rtol=0)
+    for i, (h, rb) in enumerate(zip(temps, refs)):
+        np.testing.assert_allclose(O.template_conv_closed(z, h, rb), ref[i], atol=1e-10, rtol=0)
+    np.testing.assert_allclose(O.boxcar_conv_closed(z, [1, 4]), ref[:2], atol=1e-10, rtol=0)
+    if O.good_size_real(n) % 2:
+        # the reference output is not the Ng-circular convolution at odd good sizes
+        circ = O.boxcar_conv_circular(z, [4])[0, :n]
+        assert np.abs(circ - ref[1]).max() > 0.05
+
+
+@pytest.mark.parametrize("n", [1, 3, 5, 243, 625, 675, 729])
+def test_convolve_templates_raises_at_odd_good_size_lengths(n, rng):
+    # n is its own (odd) good size: irfft returns n - 1 samples and kernels.py:905 raises
+    assert O.good_size_real(n) == n and n % 2 == 1
+    z = rng.normal(size=n)
+    with pytest.raises(ValueError):
+        O.convolve_templates_fft(z, [np.ones(1)], [0])
+    with pytest.raises(ValueError):
+        O.boxcar_conv_closed(z, [1])
+    with pytest.raises(ValueError):
+        _explicit_reference_convs(z, [np.ones(1)], [0])
+
+
+@pytest.mark.parametrize("n", [19500, 455000])
+def test_closed_form_mapping_matches_fft_restatement_large(n, rng):
+    # odd good sizes 19683 and 455625: closed form + irfft-length mapping vs the FFT restatement
+    z = rng.normal(size=n)
+    widths = [1, 8, 256]
+    fft = O.convolve_templates_fft(z, [np.ones(w) for w in widths], [0] * 3)
+    np.testing.assert_allclose(O.boxcar_conv_closed(z, widths), fft, atol=1e-9, rtol=0)
+
+
+@pytest.mark.parametrize("n", [1000, 4093, 30001, 121, 3100, 19500])
 def test_boxcar_closed_form_matches_fft(n, rng):
     z = rng.normal(size=n)
-    widths = [1, 2, 4, 8, 16, 256]
+    widths = [w for w in (1, 2, 4, 8, 16, 256) if w <= n]  # a template fits the padded series
     fft = O.convolve_templates_fft(z, [np.ones(w) for w in widths], [0] * len(widths))
     closed = O.boxcar_conv_closed(z, widths)
     np.testing.assert_allclose(closed, fft, atol=1e-9, rtol=0)
