@@ -131,6 +131,60 @@ int spp_sweep_execute_n(const spp_sweep_plan* plan, const uint8_t* packed, int64
                         void* stream);
 
 /*
+ * Batched sweep: replaces the per-call loop of Filterbank.dedisperse (base.py:405-420: one
+ * kernels.dedisperse call per gulp and DM) for many filterbanks at once. The plans (uneven
+ * nbits, nchans, nsamps and DM lists) run as one persistent kernel launch over a device work
+ * queue of (instance, DM rows, time tiles) items, stage-1 copies included; every plane equals
+ * spp_sweep_execute of its own plan bit for bit. When the stage-1 copies of all plans exceed
+ * g_budget_bytes (0: no limit) the batch is cut into rounds whose copies fit half the budget,
+ * and two pools alternate inside the same launch.
+ *
+ * Use: create -> spp_batch_get_info (workspace size) -> upload (plan tables, once) -> bind
+ * (device streams, planes and leading dimensions; again after set_shape) -> run, or capture
+ * once and replay: a bound batch reads its streams and writes its planes at the bound
+ * addresses on every run, so repeated gulps copied into the same buffers replay one graph.
+ * The plans must outlive the batch. spp_batch_errors reads and clears the device error flag
+ * (nonzero: a queue wait gave up and the planes are incomplete).
+ */
+typedef struct spp_batch spp_batch;
+
+typedef struct {
+    int64_t workspace_bytes;     /* device workspace the caller must provide            */
+    int64_t pool_bytes;          /* bytes of one stage-1 pool                           */
+    int64_t build_items;         /* stage-1 queue items                                 */
+    int64_t sweep_items;         /* stage-2 queue items                                 */
+    int32_t rounds;              /* rounds the G budget needs (1: all copies resident)   */
+    int32_t segments;            /* queue segments (2 per round)                        */
+    int32_t shape;               /* block shape (spp_batch_set_shape)                   */
+    int32_t time_tiles_per_item; /* time tiles of one stage-2 item                      */
+    int32_t dm_rows_per_item;    /* DM rows of one stage-2 item                         */
+    int32_t chans_per_stage;     /* channels per LDS stage                              */
+} spp_batch_info;
+
+int spp_batch_create(const spp_sweep_plan* const* plans, int32_t n, int64_t g_budget_bytes,
+                     spp_batch** batch);
+int spp_batch_destroy(spp_batch* batch);
+/* Block shapes (DM rows per item x channels per stage): 0 = 16 x 12 (default), 1 = 16 x 16,
+ * 2 = 8 x 8, 3 = 16 x 8; each stage-2 item covers time_tiles_per_item consecutive time tiles
+ * (2048 samples for 1/2-bit data, 1024 for 4/8-bit). All give identical planes. */
+int spp_batch_shape_count(void);
+int spp_batch_set_shape(spp_batch* batch, int shape, int32_t time_tiles_per_item);
+int spp_batch_get_info(const spp_batch* batch, spp_batch_info* info);
+/* Round of each plan (int32 [n]) into host memory. */
+int spp_batch_rounds(const spp_batch* batch, int32_t* round_of_host);
+int spp_batch_upload(spp_batch* batch, void* workspace, int64_t workspace_bytes, void* stream);
+/* packed[i]: device stream of plan i ([nsamps][nchans], nbits, its bit order); planes[i]:
+ * device plane [ndm][ld[i]], ld[i] >= the plan's max_len. Host arrays of n entries. */
+int spp_batch_bind(spp_batch* batch, const uint8_t* const* packed, float* const* planes,
+                   const int64_t* ld, void* workspace, int64_t workspace_bytes, void* stream);
+int spp_batch_run(spp_batch* batch, void* workspace, int64_t workspace_bytes, void* stream);
+/* Capture spp_batch_run into a HIP graph owned by the batch (on a private stream). */
+int spp_batch_capture(spp_batch* batch, void* workspace, int64_t workspace_bytes);
+int spp_batch_replay(spp_batch* batch, void* stream);
+int spp_batch_errors(spp_batch* batch, void* workspace, int64_t workspace_bytes, void* stream,
+                     int32_t* flags);
+
+/*
  * Gulp pre-steps (same reads and unpack as spp_dedisperse; `in` is a gulp in file order,
  * packed nbits 1/2/4/8 with bit order `big`, or float32 for nbits == 32).
  *

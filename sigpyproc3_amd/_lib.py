@@ -40,6 +40,21 @@ class SweepInfo(ctypes.Structure):
     ]
 
 
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("workspace_bytes", ctypes.c_int64),
+        ("pool_bytes", ctypes.c_int64),
+        ("build_items", ctypes.c_int64),
+        ("sweep_items", ctypes.c_int64),
+        ("rounds", ctypes.c_int32),
+        ("segments", ctypes.c_int32),
+        ("shape", ctypes.c_int32),
+        ("time_tiles_per_item", ctypes.c_int32),
+        ("dm_rows_per_item", ctypes.c_int32),
+        ("chans_per_stage", ctypes.c_int32),
+    ]
+
+
 SIGNATURES: dict[str, tuple[object, list]] = {
     "spp_last_error": (ctypes.c_char_p, []),
     "spp_version": (_int, []),
@@ -57,6 +72,18 @@ SIGNATURES: dict[str, tuple[object, list]] = {
     "spp_sweep_build": (_int, [_p, _p, _p, _i64, _p]),
     "spp_sweep_run": (_int, [_p, _p, _i64, _p, _i64, _p]),
     "spp_sweep_execute_n": (_int, [_p, _p, _i64, _p, _i64, _p, _i64, _p]),
+    "spp_batch_create": (_int, [_p, _i32, _i64, ctypes.POINTER(_p)]),
+    "spp_batch_destroy": (_int, [_p]),
+    "spp_batch_shape_count": (_int, []),
+    "spp_batch_set_shape": (_int, [_p, _int, _i32]),
+    "spp_batch_get_info": (_int, [_p, ctypes.POINTER(BatchInfo)]),
+    "spp_batch_rounds": (_int, [_p, _p]),
+    "spp_batch_upload": (_int, [_p, _p, _i64, _p]),
+    "spp_batch_bind": (_int, [_p, _p, _p, _p, _p, _i64, _p]),
+    "spp_batch_run": (_int, [_p, _p, _i64, _p]),
+    "spp_batch_capture": (_int, [_p, _p, _i64]),
+    "spp_batch_replay": (_int, [_p, _p]),
+    "spp_batch_errors": (_int, [_p, _p, _i64, _p, ctypes.POINTER(_i32)]),
     "spp_bandpass_workspace_size": (_int, [_int, _i64, _i64, _P64]),
     "spp_bandpass": (_int, [_p, _int, _int, _i64, _i64, _p, _int, _p, _i64, _p]),
     "spp_collapse": (_int, [_p, _int, _int, _i64, _i64, _p, _i64, _p]),

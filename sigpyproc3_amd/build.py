@@ -36,6 +36,7 @@ HIP_FLAGS = [
 # the same register budget.
 SOURCE_FLAGS = {
     "sweep.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
+    "batch.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
 }
 
 

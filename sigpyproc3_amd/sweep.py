@@ -22,7 +22,8 @@ DEFAULT_WIDTHS = (1, 2, 4, 8, 16, 32, 64, 128, 256)  # get_box_width_spacing(256
 class SweepPlan:
     """Host plan + device workspace of one sweep configuration."""
 
-    def __init__(self, table: np.ndarray, nsamps: int, nbits: int, big: int, *, stream=None) -> None:
+    def __init__(self, table: np.ndarray, nsamps: int, nbits: int, big: int, *, stream=None,
+                 allocate: bool = True) -> None:
         lib = _lib.load()
         _lib.require_gpu()
         self.table = np.ascontiguousarray(np.atleast_2d(table), dtype=np.int32)
@@ -40,9 +41,22 @@ class SweepPlan:
         rl = np.zeros(self.ndm, dtype=np.int64)
         _lib.check(lib.spp_sweep_plan_row_lengths(self._handle, _lib.ptr(rl)))
         self.row_len = rl
-        self.workspace = _lib.empty((int(info.workspace_bytes),), "uint8")
-        _lib.check(lib.spp_sweep_plan_upload(self._handle, _lib.ptr(self.workspace),
-                                             int(info.workspace_bytes), _lib.stream_handle(stream)))
+        self.workspace = None
+        if allocate:
+            self.allocate(stream=stream)
+
+    def allocate(self, stream=None) -> None:
+        """Allocate the device workspace (tables + stage-1 copy) and upload the tables."""
+        if self.workspace is not None:
+            return
+        nbytes = int(self.info.workspace_bytes)
+        self.workspace = _lib.empty((nbytes,), "uint8")
+        _lib.check(_lib.load().spp_sweep_plan_upload(self._handle, _lib.ptr(self.workspace), nbytes,
+                                                     _lib.stream_handle(stream)))
+
+    def release(self) -> None:
+        """Free the device workspace (a batch holds its own copy of the plan's tables)."""
+        self.workspace = None
 
     @property
     def max_len(self) -> int:

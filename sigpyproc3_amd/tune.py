@@ -1,13 +1,20 @@
-"""Autotuner for the sweep's stage-2 block shape.
+"""Autotuner for the sweep's block shapes.
 
-Each ``spp_sweep_plan`` variant (DM rows per wave x waves x channels per stage) gives the
-same bit-exact plane; their speed depends on nbits (field width), nchans and the DM
-tile spreads. ``tune_plan`` times each variant with HIP events on the caller's stream and
-keeps the fastest, caching the winner per (device, nbits, nchans, ndm, nsamps) in JSON.
+One-plan sweeps: each ``spp_sweep_plan`` variant (DM rows per wave x waves x channels per
+stage) gives the same bit-exact plane; their speed depends on nbits (field width), nchans
+and the DM tile spreads. ``tune_plan`` times each variant with HIP events on the stream the
+work runs on and keeps the fastest.
+
+Batches: ``tune_batch`` searches DM rows per item x channels per stage (the batch kernel's
+block shapes) x time tiles per item on the bound batch.
+
+Choices are cached in a JSON file keyed by the device and the exact work: nbits, nchans,
+nsamps and a hash of the delay table of every plan.
 """
 
 from __future__ import annotations
 
+import hashlib
 import json
 import os
 import threading
@@ -16,11 +23,13 @@ from pathlib import Path
 from sigpyproc3_amd import _lib
 
 _lock = threading.Lock()
-_last: dict[int, float] = {}
+_last: dict = {}
+
+BATCH_TIME_TILES = (1, 2, 4)
 
 
-def last_times() -> dict[int, float]:
-    """Per-variant stage-2 times (ms) of the most recent timed tuning."""
+def last_times() -> dict:
+    """Per-candidate times (ms) of the most recent timed tuning."""
     return dict(_last)
 
 
@@ -31,10 +40,23 @@ def cache_path() -> Path:
     return Path.home() / ".cache" / "sigpyproc3_amd" / "sweep_tune.json"
 
 
-def _key(plan) -> str:
+def _device_name() -> str:
     t = _lib.torch()
-    name = t.cuda.get_device_name() if t.cuda.is_available() else "cpu"
-    return f"{name}|nbits={plan.nbits}|nchans={plan.nchans}|ndm={plan.ndm}|nsamps={plan.nsamps}"
+    return t.cuda.get_device_name() if t.cuda.is_available() else "cpu"
+
+
+def _plan_sig(plan) -> str:
+    h = hashlib.sha1(plan.table.tobytes()).hexdigest()[:16]
+    return f"nbits={plan.nbits}|nchans={plan.nchans}|ndm={plan.ndm}|nsamps={plan.nsamps}|table={h}"
+
+
+def _key(plan) -> str:
+    return f"{_device_name()}|{_plan_sig(plan)}"
+
+
+def _batch_key(batch) -> str:
+    h = hashlib.sha1("/".join(_plan_sig(p) for p in batch.plans).encode()).hexdigest()[:24]
+    return f"{_device_name()}|batch|n={len(batch.plans)}|rounds={int(batch.info.rounds)}|{h}"
 
 
 def _load() -> dict:
@@ -53,7 +75,21 @@ def _store(entries: dict) -> None:
         tmp.write_text(json.dumps(entries, indent=1, sort_keys=True))
         os.replace(tmp, p)
     except OSError:
-        pass  # a read-only home: the choice still applies to this plan
+        pass  # a read-only home: the choice still applies to this run
+
+
+def _timed(fn, repeats: int, stream) -> float:
+    """Average ms of fn() over `repeats`, events recorded on the stream fn enqueues on."""
+    t = _lib.torch()
+    st = stream if stream is not None else t.cuda.current_stream()
+    fn()  # warm
+    e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(repeats):
+        fn()
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / repeats
 
 
 def tune_plan(plan, packed, plane=None, repeats: int = 2, cache: bool = True, stream=None) -> int:
@@ -63,21 +99,13 @@ def tune_plan(plan, packed, plane=None, repeats: int = 2, cache: bool = True, st
     if key in entries:
         plan.variant = int(entries[key]["variant"])
         return plan.variant
-    t = _lib.torch()
     if plane is None:
         plane = _lib.empty((plan.ndm, plan.max_len), "float32")
     plan.build(packed, stream=stream)
     times = {}
     for v in range(plan.variant_count()):
         plan.variant = v
-        plan.run(plane, stream=stream)  # warm
-        e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(repeats):
-            plan.run(plane, stream=stream)
-        e1.record()
-        e1.synchronize()
-        times[v] = e0.elapsed_time(e1) / repeats
+        times[v] = _timed(lambda: plan.run(plane, stream=stream), repeats, stream)
     _last.clear()
     _last.update({k: round(v, 3) for k, v in times.items()})
     best = min(times, key=times.get)
@@ -86,5 +114,35 @@ def tune_plan(plan, packed, plane=None, repeats: int = 2, cache: bool = True, st
         with _lock:
             entries = _load()
             entries[key] = {"variant": best, "ms": {str(k): round(v, 4) for k, v in times.items()}}
+            _store(entries)
+    return best
+
+
+def tune_batch(batch, repeats: int = 2, cache: bool = True, stream=None) -> tuple[int, int]:
+    """Pick (block shape, time tiles per item) for a bound batch by timing each."""
+    if batch._bound is None:
+        msg = "bind the batch before tuning it"
+        raise ValueError(msg)
+    key = _batch_key(batch)
+    with _lock:
+        entries = _load() if cache else {}
+    if key in entries:
+        batch.set_shape(int(entries[key]["shape"]), int(entries[key]["time_tiles"]), stream=stream)
+        return batch.shape
+    times = {}
+    for s in range(batch.shape_count()):
+        for tt in BATCH_TIME_TILES:
+            batch.set_shape(s, tt, stream=stream)
+            times[(s, tt)] = _timed(lambda: batch.run(stream=stream), repeats, stream)
+    batch.check(stream=stream)
+    _last.clear()
+    _last.update({f"{s}x{tt}": round(v, 3) for (s, tt), v in times.items()})
+    best = min(times, key=times.get)
+    batch.set_shape(*best, stream=stream)
+    if cache:
+        with _lock:
+            entries = _load()
+            entries[key] = {"shape": best[0], "time_tiles": best[1],
+                            "ms": {f"{s}x{tt}": round(v, 4) for (s, tt), v in times.items()}}
             _store(entries)
     return best
