@@ -24,28 +24,107 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "DM-trial·samples/s, 4096ch×2048-DM sweep (bit-exact); p50/p99 gulp latency"
 UNIT = "DM-trial·samples/s"
-VALU_ADD_PEAK_T = 78.6   # T adds/s: 256 CU x 4 SIMD x 16 lanes x 2 (packed) x 2.4 GHz
 HBM_PEAK_GBS = 8000.0
-LDS_PEAK_TBS = 78.6      # 128 B/clk/CU x 256 CU x 2.4 GHz
+# Published MI355X vector rate: 157.3 TFLOP/s FP32, where an FMA counts 2 and packed FP32
+# another 2; one plain 32-bit lane op per lane and cycle is a quarter of it.
+VALU_LANE_OPS_PEAK_T = 157.3 / 4
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def read_traffic(path: Path):
-    """HBM bytes per k_sweep dispatch from the committed rocprofv3 PMC summary, if any."""
-    if not path.exists():
-        return None
+def read_counters(path: Path) -> dict:
+    """Per-dispatch counters of k_sweep from the committed rocprofv3 PMC summary (key: value)."""
     vals = {}
-    for line in path.read_text().splitlines():
-        if ":" in line:
-            k, v = line.split(":", 1)
-            vals[k.strip()] = v.strip()
+    if path.exists():
+        for line in path.read_text().splitlines():
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                try:
+                    vals[k.strip()] = float(v.strip())
+                except ValueError:
+                    pass
+    return vals
+
+
+def lds_probe() -> dict:
+    """LDS read rates of this GPU (k_probe_lds, HIP events on its stream): TB/s of
+    ds_read_b128 (the best LDS read rate) and of ds_read2st64_b32 (the sweep's pattern)."""
+    import torch
+
+    from sigpyproc3_amd import _lib
+
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    blocks, iters = cus * 4, 20000
+    out = _lib.empty((blocks * 512,), "int32")
+    res = {}
+    for pattern, width, name in ((0, 16, "ds_read_b128"), (1, 8, "ds_read2st64_b32")):
+        best = None
+        for _ in range(4):  # the first launch warms up
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            _lib.call("spp_probe_lds", pattern, iters, blocks, _lib.ptr(out), _lib.stream_handle())
+            e1.record()
+            e1.synchronize()
+            ms = e0.elapsed_time(e1)
+            best = ms if best is None else min(best, ms)
+        nbytes = blocks * 512 * iters * 8 * width
+        res[name] = round(nbytes / (best * 1e-3) / 1e12, 2)
+    return res
+
+
+def plain_one_dm(packed, cfg, table, row_len, plane, ndm_sample: int = 32) -> dict:
+    """The plain one-DM-per-launch GPU form: spp_dedisperse (the Filterbank.dedisperse kernel)
+    once per DM over the whole resident stream, on an evenly spaced DM sample, extrapolated per
+    DM-trial sample. Its last row is compared with the sweep's plane."""
+    import torch
+
+    from sigpyproc3_amd import _lib
+
+    picks = np.linspace(0, len(table) - 1, ndm_sample).astype(int)
+    dev = _lib.device()
+    rows = [torch.from_numpy(np.ascontiguousarray(table[i])).to(dev) for i in picks]
+    mds = [int(table[i].max()) for i in picks]
+    out = _lib.empty((cfg.nsamps,), "float32")
+    big = 1 if cfg.nbits in (2, 4) else 0
+
+    def one(d, md):
+        _lib.call("spp_dedisperse", _lib.ptr(packed), cfg.nbits, big, cfg.nchans, cfg.nsamps, _lib.ptr(d), md,
+                  _lib.ptr(out), 0, _lib.stream_handle())
+
+    one(rows[0], mds[0])  # warm
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for d, md in zip(rows, mds):
+        one(d, md)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    last = int(picks[-1])
+    n = int(row_len[last])
+    exact = bool(torch.equal(out[:n], plane[last, :n]))
+    samples = float(sum(int(row_len[i]) for i in picks))
+    return {"value": samples / (ms * 1e-3), "unit": UNIT, "ms_per_dm": round(ms / ndm_sample, 3),
+            "dm_sample": f"{ndm_sample} DM rows evenly spaced over the {len(table)}, one spp_dedisperse "
+                         f"launch each over the whole {cfg.nsamps}-sample stream in HBM; extrapolated per "
+                         "DM-trial sample",
+            "last_row_bit_exact_vs_sweep": exact}
+
+
+def cpu_model() -> dict:
+    info = {}
     try:
-        return int(float(vals["fetch_bytes_per_dispatch"]) + float(vals["write_bytes_per_dispatch"]))
-    except (KeyError, ValueError):
-        return None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("model name", "cpu family", "model", "stepping", "cpu MHz") and k not in info:
+                info[k] = v.strip()
+    except OSError:
+        pass
+    info["logical_cpus_in_affinity"] = len(os.sched_getaffinity(0))
+    info["os_cpu_count"] = os.cpu_count()
+    return info
 
 
 def cpu_baseline(packed_dev, cfg, table, row_len, plane, budget_s: float):
@@ -87,12 +166,8 @@ def cpu_baseline(packed_dev, cfg, table, row_len, plane, budget_s: float):
         exact &= bool(np.array_equal(out[:n], plane[i, :n].cpu().numpy()))
         log(f"cpu baseline: DM row {i} in {t_rows[-1]:.2f} s")
     samples = float(sum(row_len[i] for i in picks))
-    secs = float(sum(t_rows)) + t_unpack * len(picks)
-    try:
-        model = next(line.split(":", 1)[1].strip() for line in open("/proc/cpuinfo")
-                     if line.startswith("model name"))
-    except (OSError, StopIteration):
-        model = "unknown"
+    t_compute = float(sum(t_rows))
+    secs = t_compute + t_unpack * len(picks)
     return {
         "value": samples / secs,
         "unit": UNIT,
@@ -101,9 +176,13 @@ def cpu_baseline(packed_dev, cfg, table, row_len, plane, budget_s: float):
         "sample": (f"{len(picks)} DM rows {picks} of the 2048, each over the full 2^20-sample stream "
                    f"(C/OpenMP restatement of kernels.dedisperse_par, {threads} threads, "
                    f"{np.mean(t_rows):.2f} s/row) plus the serial unpack of the 1 GiB packed file per "
-                   f"DM ({t_unpack:.2f} s, 1 thread) as Filterbank.dedisperse incurs it; "
-                   f"extrapolated per DM-trial sample"),
-        "cpu_model": model,
+                   f"DM ({t_unpack:.2f} s, 1 thread, timed on a 64 MiB slice and scaled) as "
+                   f"Filterbank.dedisperse incurs it; extrapolated per DM-trial sample"),
+        "compute_only": {"value": samples / t_compute, "s_per_dm": round(t_compute / len(picks), 3),
+                         "threads": threads},
+        "unpack_only": {"s_per_dm": round(t_unpack, 3), "threads": 1,
+                        "share_of_value": round(t_unpack * len(picks) / secs, 3)},
+        "cpu": cpu_model(),
         "rows_bit_exact_vs_gpu": exact,
     }
 
@@ -175,6 +254,121 @@ def bench_serve(args) -> int:
     }
     print(json.dumps(result), flush=True)
     return 0
+
+
+def bench_batch(args) -> int:
+    """The 64 uneven instances (1/2/4/8-bit, 336/1024/4096 ch, 2^16..2^19 samples, 256..4096
+    DMs) in ONE persistent launch per step, against the sum of per-instance one-plan sweeps."""
+    import torch
+
+    from sigpyproc3_amd import _lib, tune
+    from sigpyproc3_amd.batch import SweepBatch
+    from sigpyproc3_amd.sweep import SweepPlan
+    from sigpyproc3_amd.synth import cfg_batch_instance, generate_packed
+
+    _lib.load()
+    _lib.require_gpu()
+    dev = _lib.device()
+    plans, packed, desc = [], [], []
+    t0 = time.perf_counter()
+    for i in range(64):
+        cfg, dms = cfg_batch_instance(i)
+        table = cfg.delays(dms).astype(np.int32)
+        plans.append(SweepPlan(table, cfg.nsamps, cfg.nbits, 1 if cfg.nbits in (2, 4) else 0, allocate=False))
+        packed.append(generate_packed(cfg, device=dev))
+        desc.append([cfg.nbits, cfg.nchans, cfg.nsamps, len(dms)])
+        if i % 16 == 15:
+            log(f"batch: {i + 1} instances planned and generated ({time.perf_counter() - t0:.1f} s)")
+    planes = [_lib.empty((p.ndm, p.max_len), "float32") for p in plans]
+    torch.cuda.synchronize()
+    free, _ = torch.cuda.mem_get_info()
+    g_total = sum(4 * p.nchans * ((p.nsamps + 63) // 64 * 64) for p in plans)
+    budget = min(g_total, free - (12 << 30))
+    batch = SweepBatch(plans, g_budget=budget)
+    batch.bind(packed, planes)
+    log(f"batch: {batch.info.rounds} rounds of {batch.info.pool_bytes / 2**30:.1f} GiB G pools, "
+        f"{batch.info.build_items} stage-1 and {batch.info.sweep_items} stage-2 items")
+    tune_ms = None
+    if args.variant >= 0:
+        batch.set_shape(args.variant, 1)
+    else:
+        batch.tune(repeats=1, cache=False)
+        tune_ms = tune.last_times()
+        log(f"batch autotune: shape {batch.shape}; ms {tune_ms}")
+    for _ in range(args.warmup):
+        batch.run()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for e0, e1 in ev:
+        e0.record()
+        batch.run()
+        e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    errors = batch.errors()
+    launch_ms = np.array([a.elapsed_time(b) for a, b in ev])
+    total = batch.total_samples
+    sec_per_step = wall / args.steps
+    value = total / sec_per_step
+    info = batch.info
+    shape = batch.shape
+    del batch
+    torch.cuda.empty_cache()
+    log(f"batch: {sec_per_step * 1e3:.1f} ms per launch, {value:.3e} DM-trial samples/s; errors {errors}")
+    # The plain form: each instance's own one-plan sweep (stage 1 + stage 2 launches).
+    plain_ms, exact = [], True
+    for k, (p, x, pl) in enumerate(zip(plans, packed, planes)):
+        p.allocate()
+        out = p.execute(x)  # warm (and the comparison)
+        exact &= bool(torch.equal(out, pl))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        p.execute(x, out)
+        e1.record()
+        e1.synchronize()
+        plain_ms.append(e0.elapsed_time(e1))
+        del out
+        p.release()
+        if k % 16 == 15:
+            torch.cuda.empty_cache()
+    plain_s = float(sum(plain_ms)) * 1e-3
+    log(f"plain per-instance sweeps: {plain_s * 1e3:.1f} ms in total; planes bit-exact: {exact}")
+    result = {
+        "metric": "DM-trial·samples/s, 64 uneven filterbanks in one persistent launch (bit-exact)",
+        "value": value,
+        "unit": UNIT,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec_per_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "64 filterbanks: nchans {336, 1024, 4096}, nsamps U[2^16, 2^19], nbits {1, 2, 4, 8}, "
+                               "tsamp {64, 256} us, 1250-1550 MHz, DM lists of U[256, 4096] trials to DM "
+                               "U[500, 3000]; float32 DM x time planes (all resident)",
+                   "instances": desc, "dm_trial_samples": total,
+                   "g_budget_bytes": int(budget), "rounds": int(info.rounds),
+                   "pool_bytes": int(info.pool_bytes), "stage1_items": int(info.build_items),
+                   "stage2_items": int(info.sweep_items)},
+        "batch_shape": {"shape": shape[0], "time_tiles_per_item": shape[1],
+                        "dm_rows_per_item": int(info.dm_rows_per_item),
+                        "chans_per_stage": int(info.chans_per_stage)},
+        "tune_ms": tune_ms,
+        "launch_ms": {"mean": round(float(launch_ms.mean()), 3), "min": round(float(launch_ms.min()), 3)},
+        "queue_errors": errors,
+        "plain_per_instance": {"value": total / plain_s, "total_ms": round(plain_s * 1e3, 2),
+                               "note": "each instance's own SweepPlan.execute (stage 1 + stage 2, default "
+                                       "block shape), timed one after another with HIP events; planes "
+                                       "compared with the batch's"},
+        "speedup_vs_plain_per_instance": value / (total / plain_s),
+        "planes_bit_exact_vs_plain": exact,
+    }
+    print(json.dumps(result), flush=True)
+    return 0 if exact and errors == 0 else 1
 
 
 def bench_mf(args) -> int:
@@ -633,11 +827,13 @@ def main() -> int:
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="first run a plain device copy of this many GiB three times (the "
                          "reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--workload", choices=("sweep", "serve", "mf", "file", "single", "extras"), default="sweep",
+    ap.add_argument("--workload", choices=("sweep", "batch", "serve", "mf", "file", "single", "extras"),
+                    default="sweep",
                     help="sweep: the 2-bit 2048-DM sweep (the headline line); serve: the "
                          "16-stream gulp service with p50/p99 latency; mf: the 8-bit 1024-DM "
                          "sweep + normalisation + boxcar matched filter; file: the same stream "
                          "read from a SIGPROC file; single: the one-DM gulp path and unpack; "
+                         "batch: 64 uneven filterbanks in one persistent launch; "
                          "extras: the gulp pre-steps, fold and block operations")
     ap.add_argument("--serve-gulps", type=int, default=160, help="gulps per serve run")
     args = ap.parse_args()
@@ -648,6 +844,8 @@ def main() -> int:
 
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "batch":
+        return bench_batch(args)
     if args.workload == "mf":
         return bench_mf(args)
     if args.workload == "file":
@@ -734,32 +932,58 @@ def main() -> int:
     del host
 
     adds = float(total) * cfg.nchans
-    achieved_t = adds / (sweep_ms.mean() * 1e-3) / 1e12
-    traffic = read_traffic(ROOT / "profiles" / "k_sweep_2bit_4096ch_2048dm_traffic.txt")
-    roofline = {
-        "bound": "valu",
-        "achieved": round(achieved_t, 3),
-        "peak": VALU_ADD_PEAK_T,
-        "unit": "Tadd/s",
-        "frac": round(achieved_t / VALU_ADD_PEAK_T, 4),
-        "traffic": traffic,
-        "kernel": "k_sweep",
-        "kernel_ms": round(float(sweep_ms.mean()), 3),
-        "algorithmic_work": f"{total} DM-trial samples x {cfg.nchans} channel adds = {adds:.4e} adds per launch",
-    }
-    if traffic is not None:
-        roofline["hbm_gbs"] = round(traffic / (sweep_ms.mean() * 1e-3) / 1e9, 1)
-        roofline["hbm_frac"] = round(roofline["hbm_gbs"] / HBM_PEAK_GBS, 4)
+    t_sweep = float(sweep_ms.mean()) * 1e-3
+    counters = read_counters(ROOT / "profiles" / "k_sweep_2bit_4096ch_2048dm_traffic.txt")
+    traffic = (int(counters["fetch_bytes_per_dispatch"] + counters["write_bytes_per_dispatch"])
+               if "fetch_bytes_per_dispatch" in counters and "write_bytes_per_dispatch" in counters else None)
     # LDS bytes the interleaved layout reads per sample-add: one 32-bit word holds IL samples.
     il = 8 if cfg.nbits <= 2 else 4
     lds_bytes = adds * 4.0 / il
-    roofline["lds_bytes_per_launch"] = lds_bytes
-    roofline["lds_tbs"] = round(lds_bytes / (sweep_ms.mean() * 1e-3) / 1e12, 2)
-    roofline["lds_frac"] = round(roofline["lds_tbs"] / LDS_PEAK_TBS, 4)
-    roofline["note"] = ("peak = the MI355X vector rate (157.3 TFLOP/s FP32) counted as adds; the kernel "
-                        f"adds {il} packed sample fields per 32-bit lane op, so frac can exceed 1. Its "
-                        "limits are LDS reads of the interleaved words (lds_frac, vs 128 B/clk/CU) and "
-                        "VALU issue")
+    lds_tbs = lds_bytes / t_sweep / 1e12
+    probe = lds_probe()
+    log(f"LDS probe: {probe} TB/s")
+    hbm_alg = 4.0 * total + cfg.nsamps * cfg.nchans * cfg.nbits / 8
+    roofline = {
+        "bound": "lds",
+        "achieved": round(lds_tbs, 2),
+        "peak": probe["ds_read_b128"],
+        "unit": "TB/s",
+        "frac": round(lds_tbs / probe["ds_read_b128"], 4),
+        "traffic": traffic,
+        "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel and shape, scaled by a copy "
+                           "calibration: profiles/k_sweep_2bit_4096ch_2048dm_traffic.txt (not measured in "
+                           "this run)"),
+        "kernel": "k_sweep",
+        "kernel_ms": round(t_sweep * 1e3, 3),
+        "algorithmic_work": (f"{total} DM-trial samples x {cfg.nchans} channel adds = {adds:.4e} adds per launch; "
+                             f"LDS reads {lds_bytes:.4e} B (one 4-byte word per {il} sample-adds)"),
+        "peak_source": ("k_probe_lds in this run: 32 KiB of LDS per block read with ds_read_b128 (16 aligned "
+                        "bytes per lane), 4 blocks per CU"),
+        "lds_pattern_peak": probe["ds_read2st64_b32"],
+        "lds_pattern_frac": round(lds_tbs / probe["ds_read2st64_b32"], 4),
+        "lds_pattern_note": "the same probe with the sweep's own read instruction (ds_read2st64_b32)",
+        "dm_tile_reuse": {
+            "lds_bytes_per_hbm_byte": round(lds_bytes / hbm_alg, 1),
+            "algorithmic_hbm_bytes": hbm_alg,
+            "note": "LDS bytes read per algorithmic HBM byte (4 B out per DM-trial sample + the packed "
+                    "stream once); each staged window word is read by the 16 DM rows of a tile",
+        },
+        "adds": {"achieved_tadds": round(adds / t_sweep / 1e12, 1),
+                 "note": "integer sample-adds per second; one 32-bit lane op adds 8 nibble or 4 byte fields"},
+    }
+    if traffic is not None:
+        roofline["hbm"] = {"achieved_gbs": round(traffic / t_sweep / 1e9, 1), "peak_gbs": HBM_PEAK_GBS,
+                           "frac": round(traffic / t_sweep / 1e9 / HBM_PEAK_GBS, 4),
+                           "traffic_over_algorithmic": round(traffic / hbm_alg, 1)}
+    if "sq_insts_valu_per_dispatch" in counters:
+        lane_ops = counters["sq_insts_valu_per_dispatch"] * 64
+        roofline["valu"] = {"lane_ops_per_launch": lane_ops,
+                            "achieved_t_lane_ops": round(lane_ops / t_sweep / 1e12, 2),
+                            "peak_t_lane_ops": round(VALU_LANE_OPS_PEAK_T, 2),
+                            "frac": round(lane_ops / t_sweep / 1e12 / VALU_LANE_OPS_PEAK_T, 4),
+                            "source": "SQ_INSTS_VALU x 64 lanes from the committed profile, this run's kernel time"}
+    plain = plain_one_dm(packed, cfg, table, plan.row_len, plane)
+    log(f"plain one-DM-per-launch form: {plain['value']:.3e} DM-trial samples/s ({plain['ms_per_dm']} ms per DM)")
     stage1_bytes = plan.nsamps * cfg.nchans * cfg.nbits // 8 + 4 * cfg.nchans * (-(-cfg.nsamps // 64) * 64)
     result = {
         "metric": METRIC,
@@ -793,6 +1017,8 @@ def main() -> int:
         "sweep_variant": plan.variant,
         "tune_ms": tune_ms,
         "roofline": roofline,
+        "plain_one_dm_per_launch": plain,
+        "speedup_vs_plain_one_dm_per_launch": value / plain["value"],
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:

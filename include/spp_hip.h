@@ -185,6 +185,15 @@ int spp_batch_errors(spp_batch* batch, void* workspace, int64_t workspace_bytes,
                      int32_t* flags);
 
 /*
+ * LDS read-rate probe (no counterpart in the reference: the measured peak the sweep's LDS
+ * reads are compared with in the benchmark's roofline). `blocks` blocks of 512 threads read
+ * 32 KiB of LDS for iters rounds of 8 rows: pattern 0 = ds_read_b128 (16 aligned bytes per
+ * lane), pattern 1 = ds_read2st64_b32 (the sweep's window reads, 8 bytes per lane). Bytes
+ * read: blocks * 512 * iters * 8 * (16 or 8). out: uint32 [blocks * 512] (device).
+ */
+int spp_probe_lds(int pattern, int64_t iters, int32_t blocks, uint32_t* out, void* stream);
+
+/*
  * Gulp pre-steps (same reads and unpack as spp_dedisperse; `in` is a gulp in file order,
  * packed nbits 1/2/4/8 with bit order `big`, or float32 for nbits == 32).
  *

@@ -84,6 +84,7 @@ SIGNATURES: dict[str, tuple[object, list]] = {
     "spp_batch_capture": (_int, [_p, _p, _i64]),
     "spp_batch_replay": (_int, [_p, _p]),
     "spp_batch_errors": (_int, [_p, _p, _i64, _p, ctypes.POINTER(_i32)]),
+    "spp_probe_lds": (_int, [_int, _i64, _i32, _p, _p]),
     "spp_bandpass_workspace_size": (_int, [_int, _i64, _i64, _P64]),
     "spp_bandpass": (_int, [_p, _int, _int, _i64, _i64, _p, _int, _p, _i64, _p]),
     "spp_collapse": (_int, [_p, _int, _int, _i64, _i64, _p, _i64, _p]),

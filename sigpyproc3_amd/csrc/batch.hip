@@ -36,6 +36,7 @@ constexpr int BQ = 8;                       // sub-queues per segment (one per X
 constexpr int BT_RUN = 4;                   // consecutive time-tile groups per sub-queue run
 constexpr int BB_RUN = 32;                  // consecutive stage-1 time blocks per sub-queue run
 constexpr int BATCH_BCW = 32;               // packed bytes per row of a stage-1 item
+constexpr int BATCH_LOADS = 16;             // stage-1 staging loads in flight per thread
 constexpr uint32_t SPIN_LIMIT = 1u << 24;   // polls (about 0.2 us each) before a wait fails
 constexpr int NSHAPES = 4;
 
@@ -57,6 +58,10 @@ struct BInst {
 // invariant, so the compiler may repeat them where a value is used instead of holding it in
 // registers across an item (as it does with kernel arguments).
 typedef __attribute__((address_space(4))) const BInst CInst;
+
+// Read-only plan tables in the constant address space (scalar loads for uniform indices).
+typedef __attribute__((address_space(4))) const uint32_t cu32_t;
+typedef __attribute__((address_space(4))) const int32_t ci32_t;
 
 // A pointer from a descriptor, marked as global memory (a pointer loaded from memory is
 // otherwise accessed with flat instructions).
@@ -114,14 +119,23 @@ __device__ inline int find_inst(const int64_t* __restrict__ pref, int n, int64_t
     return lo;
 }
 
+// The thread index, opaque to the optimiser: values a unit derives from it (lane addresses,
+// offsets) are then computed inside the unit instead of being hoisted out of the persistent
+// loop, where they would stay live across every path of the kernel (and be spilled).
+__device__ __forceinline__ int opaque_tid() {
+    int t = static_cast<int>(threadIdx.x);
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 template <int NBITS, bool BIG>
 __device__ __forceinline__ void build_item(CInst& in, int64_t j0, int64_t b0, uint8_t* raw) {
     if (in.aligned)
-        build_tile<NBITS, BIG, true, BATCH_BCW>(glob(in.packed), in.nsamps, in.nchans, glob(in.G), in.gpitch,
-                                                in.glen, j0, b0, raw);
+        build_tile<NBITS, BIG, true, BATCH_BCW, BATCH_LOADS>(glob(in.packed), in.nsamps, in.nchans, glob(in.G), in.gpitch,
+                                                in.glen, j0, b0, raw, opaque_tid());
     else
-        build_tile<NBITS, BIG, false, BATCH_BCW>(glob(in.packed), in.nsamps, in.nchans, glob(in.G), in.gpitch,
-                                                 in.glen, j0, b0, raw);
+        build_tile<NBITS, BIG, false, BATCH_BCW, BATCH_LOADS>(glob(in.packed), in.nsamps, in.nchans, glob(in.G), in.gpitch,
+                                                 in.glen, j0, b0, raw, opaque_tid());
 }
 
 // Stage-1 unit: words [j0, j0 + B_TJ) of the packed row bytes [b0, b0 + BATCH_BCW).
@@ -134,30 +148,6 @@ __device__ __forceinline__ void run_build(CInst& in, int64_t j0, int64_t b0, uin
         case 8: build_item<4, false>(in, j0, b0, raw); break;
         case 9: build_item<4, true>(in, j0, b0, raw); break;
         default: build_item<8, false>(in, j0, b0, raw); break;
-    }
-}
-
-// Stage-2 unit: sub-tile sub of DM tile dt over time tile tt.
-template <int NBITS, bool USE32, int K, int WAVES, int CC>
-__device__ __forceinline__ void sweep_unit(CInst& in, int64_t dt, int sub, int64_t tt, uint32_t* lds) {
-    constexpr int64_t T = Fields<NBITS>::IL * SW_S;
-    sweep_tile<NBITS, USE32, K, WAVES, CC>(glob(in.G), in.gpitch, in.glen, in.nchans, glob(in.relw4), in.rpitch4,
-                                           glob(in.basew), glob(in.tile_dm0), glob(in.tile_ndm),
-                                           glob(in.tile_spread), glob(in.row_len), glob(in.plane), in.ld, 0, dt,
-                                           sub, tt * T, lds);
-}
-
-template <int K, int WAVES, int CC>
-__device__ __forceinline__ void run_sweep(CInst& in, int64_t dt, int sub, int64_t tt, uint32_t* lds) {
-    switch (in.nbits * 2 + in.use32) {
-        case 2: sweep_unit<1, false, K, WAVES, CC>(in, dt, sub, tt, lds); break;
-        case 3: sweep_unit<1, true, K, WAVES, CC>(in, dt, sub, tt, lds); break;
-        case 4: sweep_unit<2, false, K, WAVES, CC>(in, dt, sub, tt, lds); break;
-        case 5: sweep_unit<2, true, K, WAVES, CC>(in, dt, sub, tt, lds); break;
-        case 8: sweep_unit<4, false, K, WAVES, CC>(in, dt, sub, tt, lds); break;
-        case 9: sweep_unit<4, true, K, WAVES, CC>(in, dt, sub, tt, lds); break;
-        case 16: sweep_unit<8, false, K, WAVES, CC>(in, dt, sub, tt, lds); break;
-        default: sweep_unit<8, true, K, WAVES, CC>(in, dt, sub, tt, lds); break;
     }
 }
 
@@ -301,10 +291,60 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
         if (uni(Q.acquire)) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         if (!uni(Q.skip)) {
             CInst& in = *(CInst*)(A->insts + uni(Q.inst));
-            if (uni(Q.kind) == 0)
+            if (uni(Q.kind) == 0) {
                 run_build(in, uni(Q.a), uni(Q.b), reinterpret_cast<uint8_t*>(lds));
-            else
-                run_sweep<K, WAVES, CC>(in, uni(Q.a), uni(Q.sub), uni(Q.b), lds);
+            } else {
+                // Stage-2 unit: sub-tile sub of DM tile dt over time tile tt, the one-plan
+                // kernel's body for the instance's field width (included per case, so that it
+                // compiles as part of this function).
+#define SPP_TILE_EXIT break
+#define SPP_TILE_TID opaque_tid()
+                switch (in.nbits * 2 + in.use32) {
+                    case 2: {
+                        constexpr int NBITS = 1;
+                        constexpr bool USE32 = false;
+#include "spp_batch_case.inc"
+                    } break;
+                    case 3: {
+                        constexpr int NBITS = 1;
+                        constexpr bool USE32 = true;
+#include "spp_batch_case.inc"
+                    } break;
+                    case 4: {
+                        constexpr int NBITS = 2;
+                        constexpr bool USE32 = false;
+#include "spp_batch_case.inc"
+                    } break;
+                    case 5: {
+                        constexpr int NBITS = 2;
+                        constexpr bool USE32 = true;
+#include "spp_batch_case.inc"
+                    } break;
+                    case 8: {
+                        constexpr int NBITS = 4;
+                        constexpr bool USE32 = false;
+#include "spp_batch_case.inc"
+                    } break;
+                    case 9: {
+                        constexpr int NBITS = 4;
+                        constexpr bool USE32 = true;
+#include "spp_batch_case.inc"
+                    } break;
+                    case 16: {
+                        constexpr int NBITS = 8;
+                        constexpr bool USE32 = false;
+#include "spp_batch_case.inc"
+                    } break;
+                    case 17: {
+                        constexpr int NBITS = 8;
+                        constexpr bool USE32 = true;
+#include "spp_batch_case.inc"
+                    } break;
+                    default: break;
+                }
+#undef SPP_TILE_TID
+#undef SPP_TILE_EXIT
+            }
         }
         __syncthreads();  // the next unit restages the LDS this one read; Q is rewritten
     }

@@ -8,7 +8,7 @@
 // A delay shift d of channel c moves every field of a word together, so an output word
 // (the IL samples t + k*S) is the sum over channels of the aligned words G[c][t + d_i(c)].
 //
-// Stage 2 (sweep_tile): a block owns D <= 16 rows of a DM tile (<= 16 consecutive DM rows
+// Stage 2 (spp_sweep_tile.inc): a block owns D <= 16 rows of a DM tile (<= 16 consecutive DM rows
 // whose per-channel delay spread is <= SW_SPREAD_MAX, formed by the host planner) and IL*256
 // output samples. Channels are processed in stages of CC: wave w stages channels
 // w, w + WAVES, ... of the next stage, reading the windows G[c][jb .. jb + 256 + spread]
@@ -68,11 +68,12 @@ constexpr int build_lds_bytes() {
 // builds words j0 + 4q .. 4q+3 of every channel packed in byte column bc from one LDS byte
 // per (row, word) and stores them as one 16-byte write per channel (a wave writes 512
 // contiguous bytes per channel). Only the first glen words of each channel row are written
-// (the sweep reads no further). One barrier, reached by every thread of the block.
-template <int NBITS, bool BIG, bool ALIGNED, int BCW>
+// (the sweep reads no further). One barrier, reached by every thread of the block. MAXB bounds
+// the staging loads in flight per thread (registers); tid is the thread's index in the block.
+template <int NBITS, bool BIG, bool ALIGNED, int BCW, int MAXB = 32>
 __device__ __forceinline__ void build_tile(const uint8_t* __restrict__ in, int64_t nsamps, int64_t nchans,
                                            uint32_t* __restrict__ G, int64_t gpitch, int64_t glen,
-                                           int64_t j0, int64_t b0, uint8_t* raw) {
+                                           int64_t j0, int64_t b0, uint8_t* raw, int tid) {
     using F = Fields<NBITS>;
     constexpr int IL = F::IL;
     constexpr int PER = (NBITS < 8) ? 8 / NBITS : 1;     // channels per byte
@@ -84,10 +85,9 @@ __device__ __forceinline__ void build_tile(const uint8_t* __restrict__ in, int64
     constexpr int RQ = ROWS / 4 + 1;
     constexpr int STRIDE = BCW + 4;
     constexpr int NE = ROWS * QPR / B_THREADS;            // staged dwords per thread
-    constexpr int BATCH = NE < 32 ? NE : 32;              // loads in flight per thread
+    constexpr int BATCH = NE < MAXB ? NE : MAXB;          // loads in flight per thread
     static_assert(BCW % 16 == 0 && BCW <= 64, "16, 32, 48 or 64 byte columns");
     static_assert(NE % BATCH == 0 && NE * B_THREADS == ROWS * QPR, "whole batches");
-    const int tid = threadIdx.x;
     const int64_t row_bytes = nchans * NBITS / 8;
     // Stage: element e = (row r, dword q of the row's BCW bytes), loaded BATCH at a time.
 #pragma unroll 1
@@ -176,21 +176,5 @@ constexpr int kBufferFlags = 0x00020000;
 
 // Bytes 1 and 3 of x into the low bytes of its two 16-bit lanes.
 __device__ __forceinline__ uint32_t odd_bytes(uint32_t x) { return __builtin_amdgcn_perm(x, x, 0x0c030c01u); }
-
-
-// Stage 2, one block: rows [sub * D, sub * D + D) of DM tile dt (D = K * WAVES rows, K per
-// wave) and the IL * SW_S output samples from t0, with `lds` the block's 2 * CC * SW_LW words
-// of LDS. Every thread of the block calls it with the same (dt, sub, t0); its barriers are
-// reached by all of them.
-template <int NBITS, bool USE32, int K, int WAVES, int CC>
-__device__ __forceinline__ void sweep_tile(
-    const uint32_t* __restrict__ G, int64_t gpitch, int64_t glen, int64_t nchans,
-    const uint32_t* __restrict__ relw4, int64_t rpitch4, const int32_t* __restrict__ basew,
-    const int32_t* __restrict__ tile_dm0, const int32_t* __restrict__ tile_ndm,
-    const int32_t* __restrict__ tile_spread, const int32_t* __restrict__ row_len,
-    float* __restrict__ plane, int64_t ld, int64_t shrink, int64_t dt, int sub, int64_t t0,
-    uint32_t* lds) {
-#include "spp_sweep_tile.inc"
-}
 
 }  // namespace

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(B_THREADS) void k_build_g(const uint8_t* __restrict
     const int64_t logical = (b % 8) * (gridDim.x / 8) + b / 8;
     if (logical >= nby * ((glen + B_TJ - 1) / B_TJ)) return;
     build_tile<NBITS, BIG, ALIGNED, G_BCW>(in, nsamps, nchans, G, gpitch, glen, (logical / nby) * B_TJ,
-                                           (logical % nby) * G_BCW, raw);
+                                           (logical % nby) * G_BCW, raw, threadIdx.x);
 }
 
 // Stage 2: one block per (time tile, DM tile, sub-tile of K * WAVES rows). XCD-aware order:
@@ -54,7 +54,11 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
     const int64_t dt = u / NSUB;
     const int sub = static_cast<int>(u % NSUB);
     const int64_t t0 = (logical / (ntiles_dm * NSUB)) * T;
+#define SPP_TILE_EXIT return
+#define SPP_TILE_TID threadIdx.x
 #include "spp_sweep_tile.inc"
+#undef SPP_TILE_TID
+#undef SPP_TILE_EXIT
 }
 
 int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
