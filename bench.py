@@ -188,11 +188,14 @@ def cpu_baseline(packed_dev, cfg, table, row_len, plane, budget_s: float):
 
 
 def bench_serve(args) -> int:
-    """16 streams (8-bit, 4096 ch), Poisson gulp arrivals at 0.5/0.9/1.2 x measured capacity,
-    gulps of U[2^14, 2^18] new samples overlapped by each stream's largest delay, DM lists of
-    U[512, 4096] trials up to DM U[500, 3000]. Latency = arrival -> sweep complete."""
+    """16 streams (8-bit, 4096 ch), each with its own DM list of U[512, 4096] trials to DM
+    U[500, 3000]; Poisson gulp arrivals of U[2^14, 2^18] new samples at 0.5/0.9/1.2 x the
+    measured capacity, overlapped by each stream's largest delay, through the continuous-
+    batching server under an HBM budget. Latency = arrival -> sweep complete, also as a
+    fraction of the gulp's own duration; sampled rows are checked against the oracle."""
     import torch
 
+    from oracle import c_oracle
     from sigpyproc3_amd import _lib, serve
     from sigpyproc3_amd.params import compute_dmdelays
 
@@ -202,42 +205,94 @@ def bench_serve(args) -> int:
     nchans, fch1, foff, tsamp = 4096, 1550.0, -0.0732421875, 64e-6
     freqs = np.arange(nchans, dtype=np.float32) * np.float32(foff) + np.float32(fch1)
     rng = np.random.default_rng(5)
-    states, dm_lists = [], []
+    budget = int(args.serve_budget_gib * 2**30)
+    srv = serve.Server(budget, max_queued=args.serve_max_queued)
+    handles, tables = [], []
     t_plan = time.perf_counter()
     for _s in range(nstreams):
         ndm = int(rng.integers(512, 4097))
         dms = np.linspace(0.0, float(rng.uniform(500.0, 3000.0)), ndm)
         table = np.atleast_2d(compute_dmdelays(freqs, dms, tsamp, fch1)).astype(np.int32)
-        states.append(serve.StreamState(table, 8, 0, hi))
-        dm_lists.append(ndm)
+        tables.append(table)
+        handles.append(srv.open_stream(table, 8, 0, hi))
     t_plan = time.perf_counter() - t_plan
     source = serve.make_source(hi, nchans)
-    log(f"serve: {nstreams} streams, DM lists {dm_lists}, overlaps {[s.overlap for s in states]}, "
-        f"plans {t_plan:.1f} s, device memory {torch.cuda.memory_allocated() / 2**30:.1f} GiB")
-    for st in states:                       # first gulp of every stream (no overlap) + warmup
-        st.submit(source, hi)
-        st.submit(source, lo)
-    torch.cuda.synchronize()
+    src_host = source.numpy()
+    history: list[list[int]] = [[] for _ in range(nstreams)]   # accepted gulp sizes per stream
+    log(f"serve: {nstreams} streams, DM lists {[t.shape[0] for t in tables]}, overlaps "
+        f"{[h.overlap for h in handles]}, plans {t_plan:.1f} s, budget {args.serve_budget_gib} GiB")
+    # first gulp of every stream (no overlap) and one warm gulp each
+    warm = [serve.Gulp(s, hi) for s in range(nstreams)] + [serve.Gulp(s, lo) for s in range(nstreams)]
+    for g in warm:
+        history[g.stream].append(g.nsamps)
+    serve.run_arrivals(srv, handles, source, warm)
+
+    def record(g):
+        history[g.stream].append(g.nsamps)
 
     def samples_of(sched):
-        return sum(len(states[g.stream].table) * g.nsamps for g in sched)
+        return sum(tables[g.stream].shape[0] * g.nsamps for g in sched if not g.rejected)
 
     burst = serve.poisson_schedule(nstreams, args.serve_gulps, 0.0, lo, hi, seed=11)
-    wall = serve.run_arrivals(states, source, burst)
+    srv.max_queued = len(burst) + 1   # the capacity run queues every gulp
+    wall = serve.run_arrivals(srv, handles, source, burst, on_submit=record)
+    srv.max_queued = args.serve_max_queued
+    assert not any(g.rejected for g in burst)
     capacity = len(burst) / wall
     cap_samples = samples_of(burst) / wall
-    log(f"serve: capacity {capacity:.1f} gulps/s ({cap_samples:.3e} DM-trial samples/s) over {len(burst)} gulps")
+    log(f"serve: capacity {capacity:.1f} gulps/s ({cap_samples:.3e} DM-trial samples/s) over {len(burst)} "
+        f"gulps; peak budget use {srv.peak_used / 2**30:.1f} GiB")
     runs = {}
+    checks = []
     for frac in (0.5, 0.9, 1.2):
         sched = serve.poisson_schedule(nstreams, args.serve_gulps, frac * capacity, lo, hi, seed=int(frac * 100))
-        wall = serve.run_arrivals(states, source, sched)
-        summ = serve.latency_summary(sched)
+        picks = set(range(5, len(sched), max(1, len(sched) // 3)))
+
+        def sample(g, sched=sched, picks=picks):
+            k = sched.index(g)
+            if k in picks:
+                i = int(k % tables[g.stream].shape[0])
+                r = g.request
+                row = r.plane[i, int(r.row_offset[i]) : int(r.row_offset[i]) + g.nsamps].cpu().numpy()
+                checks.append((g.stream, r._hist_index, i, g.nsamps, row))
+
+        def record_idx(g):
+            g.request._hist_index = len(history[g.stream])
+            history[g.stream].append(g.nsamps)
+
+        wall = serve.run_arrivals(srv, handles, source, sched, on_done=sample, on_submit=record_idx)
+        summ = serve.latency_summary(sched, tsamp)
         summ["offered_gulps_per_s"] = frac * capacity
         summ["dm_trial_samples_per_s"] = samples_of(sched) / wall
         runs[f"{frac}x"] = summ
-        log(f"serve {frac}x: p50 {summ['p50_ms']:.1f} ms, p99 {summ['p99_ms']:.1f} ms")
+        log(f"serve {frac}x: p50 {summ['p50_ms']:.1f} ms, p99 {summ['p99_ms']:.1f} ms, latency/duration p99 "
+            f"{summ['latency_over_duration_p99']:.3f}, rejected {summ['rejected']}; queue p50/p99 "
+            f"{summ['queue_ms_p50']:.1f}/{summ['queue_ms_p99']:.1f} ms, service p50/p99 "
+            f"{summ['service_ms_p50']:.1f}/{summ['service_ms_p99']:.1f} ms")
+    # sampled rows against the oracle: each gulp's input is the stream's last `overlap` samples
+    # followed by the payload (the first n samples of the source)
+    exact = True
+    for s_, j, i, n, row in checks:
+        ov = handles[s_].overlap
+        need, parts = ov, []
+        for m in reversed(history[s_][:j]):   # the last ov samples of the previous payloads
+            take = min(m, need)
+            parts.append(src_host[(m - take) * nchans : m * nchans])
+            need -= take
+            if need == 0:
+                break
+        inp = np.concatenate(parts[::-1] + [src_host[: n * nchans]])
+        d = tables[s_][i]
+        out = np.zeros(ov + n, dtype=np.float32)
+        c_oracle.dedisperse(inp, out, d, nchans, ov + n, 0)
+        md = int(d.max())
+        exact &= bool(np.array_equal(row, out[ov - md : ov - md + n]))
+    log(f"serve: {len(checks)} sampled rows bit-exact vs the oracle: {exact}")
+    for h in handles:
+        srv.close_stream(h)
+    srv.shutdown()
     result = {
-        "metric": "p50/p99 gulp latency at 0.5/0.9/1.2 x capacity (16-stream sweep service)",
+        "metric": "p50/p99 gulp latency at 0.5/0.9/1.2 x capacity (16-stream continuous-batching sweep service)",
         "value": runs["0.9x"]["dm_trial_samples_per_s"],
         "unit": UNIT,
         "n_gpus": 1,
@@ -246,14 +301,18 @@ def bench_serve(args) -> int:
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": "16 streams x 8-bit x 4096 ch (1550 MHz, foff -0.0732, 64 us), gulps "
-                               "U[2^14, 2^18] + overlap, DM lists U[512, 4096] to DM U[500, 3000]",
-                   "dm_lists": dm_lists, "overlaps": [s.overlap for s in states]},
+                               "U[2^14, 2^18] + overlap, per-stream DM lists U[512, 4096] to DM U[500, 3000]",
+                   "dm_lists": [int(t.shape[0]) for t in tables], "overlaps": [h.overlap for h in handles],
+                   "hbm_budget_bytes": budget, "max_queued": args.serve_max_queued},
         "capacity_gulps_per_s": capacity,
         "capacity_dm_trial_samples_per_s": cap_samples,
         "latency": runs,
+        "peak_budget_bytes": srv.peak_used,
+        "sampled_rows_bit_exact_vs_oracle": exact,
+        "sampled_rows": len(checks),
     }
     print(json.dumps(result), flush=True)
-    return 0
+    return 0 if exact else 1
 
 
 def bench_batch(args) -> int:
@@ -836,6 +895,9 @@ def main() -> int:
                          "batch: 64 uneven filterbanks in one persistent launch; "
                          "extras: the gulp pre-steps, fold and block operations")
     ap.add_argument("--serve-gulps", type=int, default=160, help="gulps per serve run")
+    ap.add_argument("--serve-budget-gib", type=float, default=96.0, help="HBM budget of the serving run")
+    ap.add_argument("--serve-max-queued", type=int, default=48,
+                    help="queued gulps beyond which the server refuses new ones (backpressure)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "

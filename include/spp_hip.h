@@ -92,6 +92,7 @@ typedef struct {
     int64_t ndm_tiles;       /* DM tiles the planner formed                         */
     int32_t max_spread;      /* largest per-channel delay spread inside a DM tile   */
     int32_t mode;            /* field bits (4/8/16) + 32 if 32-bit lanes + 64*variant */
+    int64_t tables_bytes;    /* leading part of the workspace holding the plan tables */
 } spp_sweep_info;
 
 /* Plan on the host from a host delay table int32 [ndm][nchans] (all delays >= 0). */
@@ -109,7 +110,9 @@ int spp_sweep_plan_set_variant(spp_sweep_plan* plan, int variant);
 int spp_sweep_plan_info(const spp_sweep_plan* plan, spp_sweep_info* info);
 /* Row lengths (int64 [ndm]) into host memory. */
 int spp_sweep_plan_row_lengths(const spp_sweep_plan* plan, int64_t* out_host);
-/* Copy the plan's device tables into `workspace` (synchronises the stream once). */
+/* Copy the plan's device tables into `workspace` (synchronises the stream once). A workspace
+ * of tables_bytes only serves spp_sweep_execute_g; the other entry points need
+ * workspace_bytes (tables + the stage-1 copy for the plan's nsamps). */
 int spp_sweep_plan_upload(spp_sweep_plan* plan, void* workspace, int64_t workspace_bytes,
                           void* stream);
 /* Run the sweep on a packed time-major stream [nsamps][nchans] (nbits, bit order of the
@@ -129,6 +132,13 @@ int spp_sweep_run(const spp_sweep_plan* plan, float* plane, int64_t ld, void* wo
 int spp_sweep_execute_n(const spp_sweep_plan* plan, const uint8_t* packed, int64_t nsamps,
                         float* plane, int64_t ld, void* workspace, int64_t workspace_bytes,
                         void* stream);
+/* spp_sweep_execute_n with the stage-1 copy in a caller's buffer `g` of at least
+ * spp_sweep_g_bytes(plan, nsamps) bytes, pitched for nsamps: a server allocates it per gulp
+ * (and frees it when the gulp completes) while the plan tables stay resident in `tables`. */
+int spp_sweep_g_bytes(const spp_sweep_plan* plan, int64_t nsamps, int64_t* bytes);
+int spp_sweep_execute_g(const spp_sweep_plan* plan, const uint8_t* packed, int64_t nsamps,
+                        float* plane, int64_t ld, void* tables, int64_t tables_bytes,
+                        uint32_t* g, int64_t g_bytes, void* stream);
 
 /*
  * Batched sweep: replaces the per-call loop of Filterbank.dedisperse (base.py:405-420: one

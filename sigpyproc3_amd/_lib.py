@@ -37,6 +37,7 @@ class SweepInfo(ctypes.Structure):
         ("ndm_tiles", ctypes.c_int64),
         ("max_spread", ctypes.c_int32),
         ("mode", ctypes.c_int32),
+        ("tables_bytes", ctypes.c_int64),
     ]
 
 
@@ -72,6 +73,8 @@ SIGNATURES: dict[str, tuple[object, list]] = {
     "spp_sweep_build": (_int, [_p, _p, _p, _i64, _p]),
     "spp_sweep_run": (_int, [_p, _p, _i64, _p, _i64, _p]),
     "spp_sweep_execute_n": (_int, [_p, _p, _i64, _p, _i64, _p, _i64, _p]),
+    "spp_sweep_g_bytes": (_int, [_p, _i64, _P64]),
+    "spp_sweep_execute_g": (_int, [_p, _p, _i64, _p, _i64, _p, _i64, _p, _i64, _p]),
     "spp_batch_create": (_int, [_p, _i32, _i64, ctypes.POINTER(_p)]),
     "spp_batch_destroy": (_int, [_p]),
     "spp_batch_shape_count": (_int, []),

@@ -22,4 +22,5 @@ struct spp_sweep_plan {
     int64_t off_relw = 0, off_base = 0, off_dm0 = 0, off_ndm = 0, off_spread = 0, off_rowlen = 0,
             off_g = 0, ws_bytes = 0;
     const void* uploaded_to = nullptr;
+    bool uploaded_with_g = false;  // the workspace also holds the stage-1 copy (off_g .. ws_bytes)
 };

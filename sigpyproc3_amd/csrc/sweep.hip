@@ -195,6 +195,7 @@ extern "C" int spp_sweep_plan_info(const spp_sweep_plan* p, spp_sweep_info* info
     info->ndm_tiles = static_cast<int64_t>(p->tile_dm0.size());
     info->max_spread = p->max_spread;
     info->mode = (p->nbits <= 2 ? 4 : (p->nbits == 4 ? 8 : 16)) + (p->use32 ? 32 : 0) + 64 * p->variant;
+    info->tables_bytes = p->off_g;
     return SPP_OK;
 }
 
@@ -209,8 +210,8 @@ extern "C" int spp_sweep_plan_upload(spp_sweep_plan* p, void* workspace, int64_t
                                      void* stream) {
     spp::clear_error();
     SPP_REQUIRE(p != nullptr && workspace != nullptr, "null argument");
-    SPP_REQUIRE(workspace_bytes >= p->ws_bytes, "workspace too small: %lld < %lld bytes",
-                static_cast<long long>(workspace_bytes), static_cast<long long>(p->ws_bytes));
+    SPP_REQUIRE(workspace_bytes >= p->off_g, "workspace too small: %lld < %lld bytes (tables)",
+                static_cast<long long>(workspace_bytes), static_cast<long long>(p->off_g));
     hipStream_t st = spp::as_stream(stream);
     char* ws = static_cast<char*>(workspace);
     const int64_t ntiles = static_cast<int64_t>(p->tile_dm0.size());
@@ -225,35 +226,46 @@ extern "C" int spp_sweep_plan_upload(spp_sweep_plan* p, void* workspace, int64_t
     SPP_HIP(hipMemcpyAsync(ws + p->off_rowlen, p->row_len.data(), 4 * p->ndm, hipMemcpyHostToDevice, st));
     SPP_HIP(hipStreamSynchronize(st));
     p->uploaded_to = workspace;
+    p->uploaded_with_g = workspace_bytes >= p->ws_bytes;
     return SPP_OK;
 }
 
 namespace {
 
 // One execution of a plan on a stream of n <= plan nsamps samples: every row is shorter by
-// shrink = nsamps - n, and G holds glen valid words per channel.
+// shrink = nsamps - n; the stage-1 copy G (gpitch words per channel) holds glen valid words
+// per channel.
 struct Extent {
     int64_t n, glen, shrink, max_len;
+    uint32_t* G;
+    int64_t gpitch;
 };
 
-Extent extent_of(const spp_sweep_plan* p, int64_t n) {
+// G inside the plan's workspace (pitch for the plan's nsamps).
+Extent extent_of(const spp_sweep_plan* p, int64_t n, void* workspace) {
     const int64_t glen = std::min(p->gpitch, ((n + 63) / 64) * 64);
-    return Extent{n, glen, p->nsamps - n, p->max_len - (p->nsamps - n)};
+    return Extent{n, glen, p->nsamps - n, p->max_len - (p->nsamps - n),
+                  reinterpret_cast<uint32_t*>(static_cast<char*>(workspace) + p->off_g), p->gpitch};
+}
+
+// G in a caller's buffer, pitched for n samples.
+Extent extent_own_g(const spp_sweep_plan* p, int64_t n, uint32_t* G) {
+    const int64_t glen = ((n + 63) / 64) * 64;
+    return Extent{n, glen, p->nsamps - n, p->max_len - (p->nsamps - n), G, glen};
 }
 
 template <int NBITS, bool BIG>
-int launch_build(const uint8_t* packed, const spp_sweep_plan* p, const Extent& e, uint32_t* G,
-                 hipStream_t st) {
+int launch_build(const uint8_t* packed, const spp_sweep_plan* p, const Extent& e, hipStream_t st) {
     const int64_t row_bytes = p->nchans * NBITS / 8;
     const int64_t nblocks = spp::ceil_div(e.glen, int64_t(B_TJ)) * spp::ceil_div(row_bytes, int64_t(G_BCW));
     SPP_REQUIRE(nblocks + 8 <= 0x7fffffffLL, "stage-1 grid too large");
     const dim3 grid(static_cast<unsigned>(spp::ceil_div(nblocks, int64_t(8)) * 8));
     if (row_bytes % 4 == 0 && reinterpret_cast<uintptr_t>(packed) % 4 == 0)
-        hipLaunchKernelGGL((k_build_g<NBITS, BIG, true>), grid, dim3(B_THREADS), 0, st, packed, e.n, p->nchans, G,
-                           p->gpitch, e.glen);
+        hipLaunchKernelGGL((k_build_g<NBITS, BIG, true>), grid, dim3(B_THREADS), 0, st, packed, e.n, p->nchans, e.G,
+                           e.gpitch, e.glen);
     else
-        hipLaunchKernelGGL((k_build_g<NBITS, BIG, false>), grid, dim3(B_THREADS), 0, st, packed, e.n, p->nchans, G,
-                           p->gpitch, e.glen);
+        hipLaunchKernelGGL((k_build_g<NBITS, BIG, false>), grid, dim3(B_THREADS), 0, st, packed, e.n, p->nchans,
+                           e.G, e.gpitch, e.glen);
     SPP_LAUNCHED("k_build_g");
     return SPP_OK;
 }
@@ -267,8 +279,7 @@ int launch_sweep_v(const spp_sweep_plan* p, const Extent& e, char* ws, float* pl
     const int64_t grid = spp::ceil_div(nblocks, 8) * 8;
     SPP_REQUIRE(grid <= 0x7fffffffLL, "sweep grid too large");
     hipLaunchKernelGGL((k_sweep<NBITS, USE32, K, WAVES, CC, MINW>), dim3(static_cast<unsigned>(grid)),
-                       dim3(WAVES * 64), 0, st, reinterpret_cast<const uint32_t*>(ws + p->off_g),
-                       p->gpitch, e.glen, p->nchans, reinterpret_cast<const uint32_t*>(ws + p->off_relw),
+                       dim3(WAVES * 64), 0, st, e.G, e.gpitch, e.glen, p->nchans, reinterpret_cast<const uint32_t*>(ws + p->off_relw),
                        p->rpitch / 4, reinterpret_cast<const int32_t*>(ws + p->off_base),
                        reinterpret_cast<const int32_t*>(ws + p->off_dm0),
                        reinterpret_cast<const int32_t*>(ws + p->off_ndm),
@@ -309,24 +320,23 @@ int launch_sweep(const spp_sweep_plan* p, const Extent& e, char* ws, float* plan
 
 namespace {
 
-int check_exec(const spp_sweep_plan* p, const void* workspace, int64_t workspace_bytes) {
+int check_exec(const spp_sweep_plan* p, const void* workspace, int64_t workspace_bytes, bool own_g = false) {
     SPP_REQUIRE(p != nullptr && workspace != nullptr, "null argument");
     SPP_REQUIRE(workspace == p->uploaded_to, "workspace is not the buffer the plan was uploaded to");
-    SPP_REQUIRE(workspace_bytes >= p->ws_bytes, "workspace too small");
+    SPP_REQUIRE(workspace_bytes >= (own_g ? p->off_g : p->ws_bytes), "workspace too small");
+    SPP_REQUIRE(own_g || p->uploaded_with_g, "the plan was uploaded without room for its stage-1 copy");
     return SPP_OK;
 }
 
-int build_n(const spp_sweep_plan* p, const uint8_t* packed, const Extent& e, void* workspace,
-            hipStream_t st) {
-    uint32_t* G = reinterpret_cast<uint32_t*>(static_cast<char*>(workspace) + p->off_g);
+int build_n(const spp_sweep_plan* p, const uint8_t* packed, const Extent& e, hipStream_t st) {
     switch (p->nbits * 2 + p->big) {
-        case 2: return launch_build<1, false>(packed, p, e, G, st);
-        case 3: return launch_build<1, true>(packed, p, e, G, st);
-        case 4: return launch_build<2, false>(packed, p, e, G, st);
-        case 5: return launch_build<2, true>(packed, p, e, G, st);
-        case 8: return launch_build<4, false>(packed, p, e, G, st);
-        case 9: return launch_build<4, true>(packed, p, e, G, st);
-        default: return launch_build<8, false>(packed, p, e, G, st);
+        case 2: return launch_build<1, false>(packed, p, e, st);
+        case 3: return launch_build<1, true>(packed, p, e, st);
+        case 4: return launch_build<2, false>(packed, p, e, st);
+        case 5: return launch_build<2, true>(packed, p, e, st);
+        case 8: return launch_build<4, false>(packed, p, e, st);
+        case 9: return launch_build<4, true>(packed, p, e, st);
+        default: return launch_build<8, false>(packed, p, e, st);
     }
 }
 
@@ -352,7 +362,7 @@ extern "C" int spp_sweep_build(const spp_sweep_plan* p, const uint8_t* packed, v
     int rc = check_exec(p, workspace, workspace_bytes);
     if (rc) return rc;
     SPP_REQUIRE(packed != nullptr, "null packed stream");
-    return build_n(p, packed, extent_of(p, p->nsamps), workspace, spp::as_stream(stream));
+    return build_n(p, packed, extent_of(p, p->nsamps, workspace), spp::as_stream(stream));
 }
 
 extern "C" int spp_sweep_run(const spp_sweep_plan* p, float* plane, int64_t ld, void* workspace,
@@ -360,7 +370,7 @@ extern "C" int spp_sweep_run(const spp_sweep_plan* p, float* plane, int64_t ld, 
     spp::clear_error();
     int rc = check_exec(p, workspace, workspace_bytes);
     if (rc) return rc;
-    return run_n(p, extent_of(p, p->nsamps), plane, ld, workspace, spp::as_stream(stream));
+    return run_n(p, extent_of(p, p->nsamps, workspace), plane, ld, workspace, spp::as_stream(stream));
 }
 
 extern "C" int spp_sweep_execute(const spp_sweep_plan* p, const uint8_t* packed, float* plane,
@@ -383,9 +393,39 @@ extern "C" int spp_sweep_execute_n(const spp_sweep_plan* p, const uint8_t* packe
     SPP_REQUIRE(nsamps > p->nsamps - p->min_len,
                 "nsamps %lld must exceed the largest delay of the table (%lld)",
                 static_cast<long long>(nsamps), static_cast<long long>(p->nsamps - p->min_len));
-    const Extent e = extent_of(p, nsamps);
+    const Extent e = extent_of(p, nsamps, workspace);
     hipStream_t st = spp::as_stream(stream);
-    rc = build_n(p, packed, e, workspace, st);
+    rc = build_n(p, packed, e, st);
     if (rc) return rc;
     return run_n(p, e, plane, ld, workspace, st);
+}
+
+extern "C" int spp_sweep_g_bytes(const spp_sweep_plan* p, int64_t nsamps, int64_t* bytes) {
+    spp::clear_error();
+    SPP_REQUIRE(p != nullptr && bytes != nullptr, "null argument");
+    SPP_REQUIRE(nsamps > 0 && nsamps <= p->nsamps, "nsamps %lld outside (0, %lld]", static_cast<long long>(nsamps),
+                static_cast<long long>(p->nsamps));
+    *bytes = 4 * p->nchans * (((nsamps + 63) / 64) * 64);
+    return SPP_OK;
+}
+
+extern "C" int spp_sweep_execute_g(const spp_sweep_plan* p, const uint8_t* packed, int64_t nsamps, float* plane,
+                                   int64_t ld, void* tables, int64_t tables_bytes, uint32_t* g, int64_t g_bytes,
+                                   void* stream) {
+    spp::clear_error();
+    int rc = check_exec(p, tables, tables_bytes, true);
+    if (rc) return rc;
+    SPP_REQUIRE(packed != nullptr && g != nullptr, "null argument");
+    SPP_REQUIRE(nsamps <= p->nsamps, "nsamps %lld exceeds the plan's %lld", static_cast<long long>(nsamps),
+                static_cast<long long>(p->nsamps));
+    SPP_REQUIRE(nsamps > p->nsamps - p->min_len,
+                "nsamps %lld must exceed the largest delay of the table (%lld)",
+                static_cast<long long>(nsamps), static_cast<long long>(p->nsamps - p->min_len));
+    const Extent e = extent_own_g(p, nsamps, g);
+    SPP_REQUIRE(g_bytes >= 4 * p->nchans * e.gpitch, "stage-1 buffer too small: %lld < %lld bytes",
+                static_cast<long long>(g_bytes), static_cast<long long>(4 * p->nchans * e.gpitch));
+    hipStream_t st = spp::as_stream(stream);
+    rc = build_n(p, packed, e, st);
+    if (rc) return rc;
+    return run_n(p, e, plane, ld, tables, st);
 }

@@ -605,51 +605,6 @@ def test_fold_vs_oracle(filfiles, key):
         fil.fold(0.01, 0.0, nbins=4096, nints=4096, nbands=64)
 
 
-# ---------------------------------------------------------------------------
-# streaming service: overlapping gulps of varying length
-# ---------------------------------------------------------------------------
-
-
-@pytest.mark.parametrize("nbits", [2, 8])
-def test_serve_gulps_continue_rows(nbits, rng):
-    """Gulps of different lengths through one StreamState: the new samples of each row equal
-    the one-shot dedispersion of the whole stream at the same absolute times."""
-    import torch
-
-    from sigpyproc3_amd import _lib, serve
-
-    nchans = 128
-    order = O.DEFAULT_BITORDER.get(nbits, "big")
-    dms = np.linspace(0, 60, 24)
-    f = O.chan_freqs(nchans, -1.5, 1500.0)
-    table = np.atleast_2d(O.compute_dmdelays(f, dms, 64e-6, 1500.0)).astype(np.int32)
-    sizes = [9000, 3001, 6500]
-    total = sum(sizes)
-    x = rng.integers(0, 1 << nbits, (total, nchans), dtype=np.uint8)
-    packed = x.reshape(-1) if nbits == 8 else O.pack(x.reshape(-1), nbits, order)
-    rb = nchans * nbits // 8
-    st = serve.StreamState(table, nbits, 1 if order == "big" else 0, max(sizes))
-    md = table.max(axis=1).astype(np.int64)
-    ref = c_oracle.sweep(x.reshape(-1), nchans, table, ld=total)    # rows of the whole stream
-    start = 0
-    for k, n in enumerate(sizes):
-        src = torch.from_numpy(np.ascontiguousarray(packed[start * rb : (start + n) * rb])).pin_memory()
-        plane, ld, off, ev = st.submit(src, n)
-        ev.synchronize()
-        got = plane[: table.shape[0] * ld].reshape(table.shape[0], ld).cpu().numpy()
-        for i in range(table.shape[0]):
-            if k == 0:                       # first gulp: rows start at absolute time 0
-                t_abs, lo, cnt = 0, 0, n - md[i]
-            else:                            # later gulps: n new samples per row
-                t_abs, lo, cnt = start - md[i], int(off[i]), n
-            np.testing.assert_array_equal(got[i, lo : lo + cnt], ref[i, t_abs : t_abs + cnt],
-                                          err_msg=f"gulp {k} row {i}")
-        start += n
-    assert start == total
-    del st
-    _lib.torch().cuda.synchronize()
-
-
 def test_gulp_loop_graph_replays_match_eager(rng):
     """The one-DM gulp loop over resident data captured as a HIP graph: three replays, with
     eager work and synchronisation between them, each equal bit for bit to the eager pass

@@ -311,3 +311,60 @@ def test_read_plan_packed_into_alternates_buffers(filfiles, skipback):
     for (n1, i1, r1), (n2, i2, r2) in zip(ref, got):
         assert (n1, i1) == (n2, i2)
         np.testing.assert_array_equal(r1, r2)
+
+
+def test_serve_admission_is_fifo_and_bounded():
+    """The server's admission policy: arrival order, no overtaking, never over budget."""
+    import collections
+    import types
+
+    from sigpyproc3_amd.serve import Arena, admit_in_order
+
+    a = Arena(100 * 256)
+    q = collections.deque(types.SimpleNamespace(id=i, need=n * 256) for i, n in enumerate([40, 50, 30, 5, 90]))
+    first = admit_in_order(q, a)
+    assert [r.id for r in first] == [0, 1] and a.used == 90 * 256
+    assert admit_in_order(q, a) == []          # 30 does not fit; 5 behind it may not overtake
+    a.free(first[0].offset, 40 * 256)
+    got = admit_in_order(q, a)
+    assert [r.id for r in got] == [2, 3] and a.used == 85 * 256
+    for r in [first[1]] + got:
+        a.free(r.offset, r.need)
+    assert a.used == 0 and a.free_list == [[0, 100 * 256]]   # freed ranges coalesce
+    (last,) = admit_in_order(q, a)
+    assert last.id == 4
+    a.free(last.offset, last.need)
+    with pytest.raises(RuntimeError):
+        a.free(last.offset, 256)               # already free
+
+
+def test_serve_admission_one_gulp_per_stream():
+    import collections
+    import types
+
+    from sigpyproc3_amd.serve import Arena, admit_in_order
+
+    a = Arena(1000 * 256)
+    q = collections.deque(types.SimpleNamespace(id=i, need=10 * 256, handle=h)
+                          for i, h in enumerate(["s0", "s1", "s0", "s2", "s1"]))
+    busy = set()
+    got = admit_in_order(q, a, busy)
+    assert [r.id for r in got] == [0, 1, 3] and busy == {"s0", "s1", "s2"}
+    assert [r.id for r in q] == [2, 4]          # their streams' earlier gulps are on the device
+    busy.discard("s0")
+    assert [r.id for r in admit_in_order(q, a, busy)] == [2]
+    assert [r.id for r in q] == [4]
+
+
+def test_serve_arena_fragmentation():
+    from sigpyproc3_amd.serve import Arena
+
+    a = Arena(10 * 256)
+    offs = [a.alloc(256) for _ in range(10)]
+    assert offs == [256 * i for i in range(10)] and a.alloc(1) is None
+    for o in offs[::2]:
+        a.free(o, 256)
+    assert a.used == 5 * 256 and a.alloc(512) is None   # free space is fragmented
+    a.free(offs[1], 256)
+    assert a.alloc(512) == 0                              # 0 and 1 coalesced (2 is free too)
+    assert a.peak == 10 * 256
