@@ -321,6 +321,16 @@ int spp_boxcar_mf_workspace_size(int64_t nrows, const int64_t* row_len_host,
 int spp_boxcar_mf(const float* z, int64_t ldz, int64_t nrows, const int64_t* row_len_host,
                   const int32_t* widths_host, int32_t nwidths, float* snr, int32_t* peak,
                   int32_t* widx, void* workspace, int64_t workspace_bytes, void* stream);
+/* spp_boxcar_mf without a workspace, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops). No counterpart in the reference. Its scratch
+ * is a static device array of 8 MiB per GPU held until the process ends (enough for 1024 rows
+ * of 2^19 samples with the 9 widths 1..256); rows that need more are refused with
+ * SPP_EINVAL. Calls no device allocator. Calls that overlap on one GPU share the scratch and
+ * race: serialise them. Nothing in the scratch needs clearing (each call writes every slot
+ * it reads). Synchronises the stream. */
+int spp_boxcar_mf_noalloc(const float* z, int64_t ldz, int64_t nrows, const int64_t* row_len_host,
+                          const int32_t* widths_host, int32_t nwidths, float* snr, int32_t* peak,
+                          int32_t* widx, void* stream);
 
 /*
  * Template bank convolution. Replaces kernels.convolve_templates (core/kernels.py:861-906)

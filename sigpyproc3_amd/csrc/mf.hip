@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -708,6 +709,53 @@ extern "C" int spp_boxcar_mf(const float* z, int64_t ldz, int64_t nrows, const i
     SPP_LAUNCHED("k_boxcar_final");
     SPP_HIP(hipStreamSynchronize(st));
     return SPP_OK;
+}
+
+namespace {
+
+// Scratch of the allocation-free matched filter (spp_boxcar_mf_noalloc): a file-scope device
+// array, one per GPU, held until the process ends. Host code never passes the array's name
+// to a kernel (on the host it is only a handle); it uses the device address that
+// hipGetSymbolAddress returns on each device, looked up once per device.
+constexpr int64_t MF_SCRATCH_BYTES = int64_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[MF_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // per device id
+
+int scratch_address(void** out) {
+    int dev = 0;
+    SPP_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev >= static_cast<int>(g_scratch_ptr.size())) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        SPP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        SPP_REQUIRE(p != nullptr, "no device address for the matched-filter scratch on device %d", dev);
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SPP_OK;
+}
+
+}  // namespace
+
+extern "C" int spp_boxcar_mf_noalloc(const float* z, int64_t ldz, int64_t nrows, const int64_t* row_len_host,
+                                     const int32_t* widths_host, int32_t nwidths, float* snr, int32_t* peak,
+                                     int32_t* widx, void* stream) {
+    spp::clear_error();
+    int64_t need = 0;
+    int rc = spp_boxcar_mf_workspace_size(nrows, row_len_host, widths_host, nwidths, &need);
+    if (rc != SPP_OK) return rc;
+    SPP_REQUIRE(need <= MF_SCRATCH_BYTES,
+                "these rows need %lld bytes of matched-filter scratch; the allocation-free entry point "
+                "holds %lld (spp_boxcar_mf takes a workspace of any size)",
+                static_cast<long long>(need), static_cast<long long>(MF_SCRATCH_BYTES));
+    void* ws = nullptr;
+    rc = scratch_address(&ws);
+    if (rc != SPP_OK) return rc;
+    return spp_boxcar_mf(z, ldz, nrows, row_len_host, widths_host, nwidths, snr, peak, widx, ws, MF_SCRATCH_BYTES,
+                         stream);
 }
 
 namespace {
