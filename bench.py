@@ -548,6 +548,24 @@ def bench_single(args) -> int:
     bytes_pass = sum(n * cfg.nchans for _, n, _ in starts) + 4 * (cfg.nsamps - md)
     row = tim.cpu().numpy()
     peak_t = int(np.argmax(row))
+    # The stream resident in HBM as a whole: one launch over every output sample (what
+    # sweep.dedisperse_resident does for non-negative delays), compared with the gulp pass.
+    tim1 = _lib.zeros((cfg.nsamps - md,), "float32")
+
+    def one_launch(st=stream):
+        _lib.call("spp_dedisperse", _lib.ptr(packed), 8, 0, cfg.nchans, cfg.nsamps, _lib.ptr(d_dev), md,
+                  _lib.ptr(tim1), 0, st)
+
+    one_launch()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        one_launch()
+    e1.record()
+    torch.cuda.synchronize()
+    t_one = e0.elapsed_time(e1) / reps * 1e-3
+    one_equal = bool(np.array_equal(tim1.cpu().numpy(), row))
+    bytes_one = cfg.nsamps * cfg.nchans + 4 * (cfg.nsamps - md)
     # The same pass captured once as a HIP graph and replayed: the gulp launches without the
     # host's per-call cost (the kernels allocate nothing and clear nothing).
     graph = torch.cuda.CUDAGraph()
@@ -621,12 +639,15 @@ def bench_single(args) -> int:
                        "hbm_gbs": bytes_pass / t_pass / 1e9, "hbm_frac": bytes_pass / t_pass / 8e12,
                        "peak_sample": peak_t, "pulse_at": [20000, 20015],
                        "graph_us_per_pass": t_graph * 1e6, "graph_hbm_gbs": bytes_pass / t_graph / 1e9,
-                       "graph_output_equal": graph_equal},
+                       "graph_output_equal": graph_equal,
+                       "resident_one_launch_us": t_one * 1e6, "resident_hbm_gbs": bytes_one / t_one / 1e9,
+                       "resident_hbm_frac": bytes_one / t_one / 8e12, "resident_output_equal": one_equal},
         "dedisperse_2bit_4096ch_2p18": wide,
         "unpack_2bit": {"ms": t_unpack * 1e3, "bytes": nb * 5, "hbm_gbs": nb * 5 / t_unpack / 1e9,
                         "hbm_frac": nb * 5 / t_unpack / 8e12},
     }
-    log(f"single: {t_pass * 1e6:.1f} us per dedisperse pass ({result['dedisperse']['hbm_gbs']:.0f} GB/s), "
+    log(f"single: resident stream in one launch {t_one * 1e6:.1f} us (equal: {one_equal}); "
+        f"{t_pass * 1e6:.1f} us per dedisperse pass ({result['dedisperse']['hbm_gbs']:.0f} GB/s), "
         f"{t_graph * 1e6:.1f} us replayed as a graph (output equal: {graph_equal}); 2-bit 4096 ch gulp "
         + ", ".join(f"{k} {v['us']:.0f} us" for k, v in wide.items()) + "; "
         f"unpack {result['unpack_2bit']['hbm_gbs']:.0f} GB/s")

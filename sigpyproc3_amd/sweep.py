@@ -182,6 +182,56 @@ def load_packed_stream(fil, start: int, nsamps: int):
     return host.to(_lib.device(), non_blocking=False)
 
 
+def dedisperse_resident(packed, header, dm: float, ref_freq: str | float = "fch1", *, stream=None):
+    """``Filterbank.dedisperse(dm, ref_freq)`` (base.py:360-424) of a whole stream already in
+    HBM (``packed``: the file's packed samples, e.g. from ``load_packed_stream``), as one
+    ``spp_dedisperse`` launch over the stream instead of one per gulp.
+
+    With non-negative delays every output sample sums the same inputs whatever the gulp
+    schedule, so one launch gives the reference's time series bit for bit. Negative delays
+    (an ascending band, or a reference frequency below the top) read before each gulp and
+    wrap inside it (numba's negative indexing, kernels.py:361), so those keep the reference's
+    gulp schedule (gulp 16384, skip-back of the largest delay) over the resident bytes.
+    """
+    from sigpyproc3_amd.readers import plan_blocks
+    from sigpyproc3_amd.timeseries import TimeSeries
+
+    _lib.require_gpu()
+    hdr = header
+    nbits = int(hdr.nbits)
+    if nbits not in (1, 2, 4, 8):
+        msg = "dedisperse_resident takes 1/2/4/8-bit data"
+        raise ValueError(msg)
+    big = 1 if nbits in (2, 4) else 0
+    delays = np.ascontiguousarray(hdr.get_dmdelays(dm, ref_freq), dtype=np.int32)
+    md = int(np.abs(delays).max())
+    n = int(hdr.nsamples)
+    tim_len = n - md
+    if tim_len < 0:
+        msg = "negative dimensions are not allowed"
+        raise ValueError(msg)
+    row_bytes = hdr.nchans * nbits // 8
+    if int(packed.numel()) < n * row_bytes:
+        msg = f"packed stream holds {int(packed.numel())} bytes < {n * row_bytes}"
+        raise ValueError(msg)
+    tim = _lib.zeros((max(tim_len, 1),), "float32")
+    d_dev = _lib.to_device(delays)
+    st = _lib.stream_handle(stream)
+    if tim_len > 0 and int(delays.min()) >= 0:
+        _lib.call("spp_dedisperse", _lib.ptr(packed), nbits, big, hdr.nchans, n, _lib.ptr(d_dev), md,
+                  _lib.ptr(tim), 0, st)
+    elif tim_len > 0:
+        gulp = max(2 * md, 16384)
+        _, blocks = plan_blocks(n, gulp, md)
+        pos = 0
+        for ii, nr, skip in blocks:
+            _lib.call("spp_dedisperse", _lib.ptr(packed[pos * row_bytes :]), nbits, big, hdr.nchans, nr,
+                      _lib.ptr(d_dev), md, _lib.ptr(tim), ii * (gulp - md), st)
+            pos += nr + skip
+    data = tim[:tim_len].cpu().numpy() if tim_len > 0 else np.zeros(0, dtype=np.float32)
+    return TimeSeries(data, hdr.new_header({"nchans": 1, "dm": dm, "nsamples": tim_len}))
+
+
 def running_normalise(x, row_len, window: int = 4096, *, ld: int | None = None, stream=None):
     """Running mean/std normalisation of each row (``spp_running_normalise``).
 

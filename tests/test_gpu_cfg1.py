@@ -47,3 +47,28 @@ def test_single_dm_dedisperse_8bit_1024ch(tmp_path):
     peak = int(np.argmax(tim.data))
     assert 20000 <= peak < 20016
     assert tim.data[20000:20016].min() > np.median(tim.data) + 10 * 1024
+
+
+def test_resident_stream_one_launch_equals_gulp_path(tmp_path):
+    """The same file's stream resident in HBM, dedispersed in one launch, equals the gulp
+    path; with mixed-sign delays (reference frequency mid-band) the resident path keeps the
+    reference's gulp schedule and still matches."""
+    import torch
+
+    from sigpyproc3_amd import FilReader, synth
+    from sigpyproc3_amd.sweep import dedisperse_resident, load_packed_stream
+
+    cfg = synth.cfg_single_dm()
+    packed = synth.generate_packed(cfg, device="cpu")
+    path = tmp_path / "single_dm_8bit_1024ch.fil"
+    synth.write_sigproc(path, cfg, packed)
+    fil = FilReader(str(path))
+    dev = load_packed_stream(fil, 0, fil.header.nsamples)
+    for ref in ("fch1", "fcenter"):
+        print(f"resident one-launch vs gulp path, ref_freq={ref}", flush=True)
+        a = fil.dedisperse(300.0, ref_freq=ref)
+        b = dedisperse_resident(dev, fil.header, 300.0, ref_freq=ref)
+        torch.cuda.synchronize()
+        assert a.data.shape == b.data.shape
+        np.testing.assert_array_equal(a.data, b.data)
+        assert b.header.nsamples == a.header.nsamples
