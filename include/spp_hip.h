@@ -333,6 +333,27 @@ int spp_boxcar_mf_noalloc(const float* z, int64_t ldz, int64_t nrows, const int6
                           int32_t* widx, void* stream);
 
 /*
+ * Running normalisation and boxcar MF in one pass over the rows of x: the MatchedFilter of
+ * core/filters.py:325-338 (convolve_templates, core/kernels.py:861-906, with boxcar
+ * templates) applied to the running mean/std normalisation of each row
+ * (core/stats.py:231-236 padding), as spp_boxcar_mf(spp_running_normalise(x)) but without
+ * the normalised plane in device memory. z is computed with spp_running_normalise's
+ * arithmetic; S and the box sums in float64. Each width's first maximum is taken on the
+ * float64 box sums, then conv is rounded to float32 and the template-major first maximum
+ * kept, so (snr, peak, widx) equal spp_boxcar_mf's except where float32 rounding makes two
+ * box sums of one width equal. Rows whose good size is odd take spp_boxcar_mf's FFT path
+ * (one row at a time); a row of exactly Ng (odd) samples is refused with SPP_EINVAL.
+ * Row lengths 1 <= n <= min(ldx, 2^31 - 1) and n >= the largest width; window >= 1.
+ * workspace: spp_normalise_boxcar_mf_workspace_size bytes. Synchronises the stream.
+ */
+int spp_normalise_boxcar_mf_workspace_size(int64_t nrows, const int64_t* row_len_host,
+                                           const int32_t* widths_host, int32_t nwidths, int64_t* bytes);
+int spp_normalise_boxcar_mf(const float* x, int64_t ldx, int64_t nrows, const int64_t* row_len_host,
+                            int32_t window, const int32_t* widths_host, int32_t nwidths, float* snr,
+                            int32_t* peak, int32_t* widx, void* workspace, int64_t workspace_bytes,
+                            void* stream);
+
+/*
  * Template bank convolution. Replaces kernels.convolve_templates (core/kernels.py:861-906)
  * for any template: convs[k][t] = (sum_m h_k[m] zt[(t - ref_k + m) mod Ng] - mu_k*S) /
  * sigma_k with mu_k = sum(h_k)/Ng and sigma_k = sqrt(sum(h_k^2) - sum(h_k)^2/Ng) (the

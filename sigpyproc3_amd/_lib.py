@@ -103,6 +103,8 @@ SIGNATURES: dict[str, tuple[object, list]] = {
     "spp_boxcar_mf_workspace_size": (_int, [_i64, _p, _p, _i32, _P64]),
     "spp_boxcar_mf": (_int, [_p, _i64, _i64, _p, _p, _i32, _p, _p, _p, _p, _i64, _p]),
     "spp_boxcar_mf_noalloc": (_int, [_p, _i64, _i64, _p, _p, _i32, _p, _p, _p, _p]),
+    "spp_normalise_boxcar_mf_workspace_size": (_int, [_i64, _p, _p, _i32, _P64]),
+    "spp_normalise_boxcar_mf": (_int, [_p, _i64, _i64, _p, _i32, _p, _i32, _p, _p, _p, _p, _i64, _p]),
     "spp_convolve_templates_workspace_size": (_int, [_i64, _i32, _i64, _P64]),
     "spp_convolve_templates": (_int, [_p, _i64, _p, _p, _p, _i32, _p, _p, _i64, _p]),
 }

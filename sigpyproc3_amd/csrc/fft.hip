@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace {
 
@@ -304,14 +305,24 @@ namespace spp {
 
 int64_t good_size_real(int64_t n) {
     if (n <= 6) return n;
-    int64_t best = 2 * n;
-    for (int64_t f5 = 1; f5 < best; f5 *= 5)
-        for (int64_t f35 = f5; f35 < best; f35 *= 3) {
-            int64_t x = f35;
-            while (x < n) x *= 2;
-            best = std::min(best, x);
+    // all 2^a 3^b 5^c up to 2^62, sorted, built once: the smallest one >= n
+    static const std::vector<int64_t> smooth = [] {
+        std::vector<int64_t> v;
+        constexpr int64_t kMax = int64_t(1) << 62;
+        for (int64_t f5 = 1; f5 <= kMax; f5 *= 5) {
+            for (int64_t f35 = f5; f35 <= kMax; f35 *= 3) {
+                for (int64_t x = f35; x <= kMax; x *= 2) {
+                    v.push_back(x);
+                    if (x > kMax / 2) break;
+                }
+                if (f35 > kMax / 3) break;
+            }
+            if (f5 > kMax / 5) break;
         }
-    return best;
+        std::sort(v.begin(), v.end());
+        return v;
+    }();
+    return *std::lower_bound(smooth.begin(), smooth.end(), n);
 }
 
 int32_t odd_conv_batch(int64_t n, int32_t ntemps) {

@@ -252,6 +252,30 @@ def running_normalise(x, row_len, window: int = 4096, *, ld: int | None = None, 
     return z.cpu().numpy() if host else z
 
 
+def normalise_boxcar_mf(x, row_len, window: int = 4096, widths=DEFAULT_WIDTHS, *, stream=None):
+    """``boxcar_mf(running_normalise(x, row_len, window), row_len, widths)`` in one pass over
+    ``x`` (``spp_normalise_boxcar_mf``): the normalised series stays on chip.
+
+    Returns host arrays ``(snr float32, peak int32, width_index int32)``.
+    """
+    xd = _lib.to_device(np.ascontiguousarray(x, dtype=np.float32)) if isinstance(x, np.ndarray) else x
+    if not xd.is_contiguous():
+        xd = xd.contiguous()
+    nrows, ldx = int(xd.shape[0]), int(xd.shape[1])
+    rl = np.ascontiguousarray(row_len, dtype=np.int64)
+    w = np.ascontiguousarray(widths, dtype=np.int32)
+    need = np.zeros(1, dtype=np.int64)
+    _lib.call("spp_normalise_boxcar_mf_workspace_size", nrows, _lib.ptr(rl), _lib.ptr(w), int(w.size),
+              need.ctypes.data_as(_lib._P64))
+    ws = _lib.empty((int(need[0]),), "uint8")
+    out = _lib.empty((3, nrows), "int32")  # float32 snr bits, peak, width index
+    _lib.call("spp_normalise_boxcar_mf", _lib.ptr(xd), ldx, nrows, _lib.ptr(rl), int(window), _lib.ptr(w),
+              int(w.size), _lib.ptr(out[0]), _lib.ptr(out[1]), _lib.ptr(out[2]), _lib.ptr(ws), int(need[0]),
+              _lib.stream_handle(stream))
+    host = out.cpu().numpy()
+    return host[0].view(np.float32).copy(), host[1].copy(), host[2].copy()
+
+
 def boxcar_mf(z, row_len, widths=DEFAULT_WIDTHS, *, stream=None):
     """Boxcar matched filter + per-row argmax (``spp_boxcar_mf``).
 
@@ -301,12 +325,12 @@ class SweepResult:
                                                                 "nsamples": n}))
 
     def matched_filter(self, window: int = 4096, widths=DEFAULT_WIDTHS):
-        """Running normalisation over ``window`` samples, then boxcar MF per DM row.
+        """Running normalisation over ``window`` samples, then boxcar MF per DM row, in one
+        pass over the plane (``normalise_boxcar_mf``).
 
         Returns ``(snr, peak_bin, width_index)`` host arrays of length ndm.
         """
-        z = running_normalise(self.plane, self.row_len, window)
-        return boxcar_mf(z, self.row_len, widths)
+        return normalise_boxcar_mf(self.plane, self.row_len, window, widths)
 
 
 def mf_good_sizes(row_len) -> np.ndarray:
