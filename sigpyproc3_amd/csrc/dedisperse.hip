@@ -15,6 +15,7 @@
 // bound by instruction issue (a byte write per sample staged, a byte read and an add per
 // sample summed), not by HBM.
 #include "spp_common.hpp"
+#include "spp_gulp_swar.hpp"
 #include "spp_unpack.hpp"
 
 #include <algorithm>
@@ -372,10 +373,56 @@ int fill_f32(float* p, int64_t n, hipStream_t st) {
     return SPP_OK;
 }
 
+// The dedisperse form with word-parallel adds (gulp_swar.hip), for rows of whole dwords.
+// Channels are grouped as below, with the SWAR kernel's stage (a fixed run of dwords of each
+// row) in place of DD_CC channels and its own time tile.
+int launch_swar(const uint8_t* in, int nbits, int big, const int32_t* delays, float* out, int64_t nchans,
+                int64_t nsamps, int64_t nout, int64_t out_start, hipStream_t st) {
+    const int64_t row_bytes = nchans * nbits / 8;
+    const int64_t T = spp::gulp_swar_tile(nbits);
+    const int64_t chunk = spp::gulp_swar_stage_channels(nbits);  // channels of one stage
+    const int64_t bx = spp::ceil_div(nout, T);
+    SPP_REQUIRE(bx <= 0x7fffffffLL, "gulp too long");
+    const int64_t cus = spp::device_cu_count();
+    const int64_t maxval = (int64_t(1) << nbits) - 1;
+    const bool exact = nchans * maxval < (int64_t(1) << 24);
+    int64_t groups = 1;
+    if (exact)
+        groups = std::max<int64_t>(1, std::min<int64_t>(spp::ceil_div(8 * cus, bx), spp::ceil_div(nchans, chunk)));
+    // 1/2-bit counters are 16 bits wide: at most 21 760 channels per block
+    if (nbits <= 2) groups = std::max<int64_t>(groups, spp::ceil_div(nchans, int64_t(21760)));
+    int64_t cpg = spp::ceil_div(spp::ceil_div(nchans, groups), chunk) * chunk;
+    int64_t ilv = 1, nchunk = 1;
+    const int64_t line_ch = 1024 / nbits;  // channels in a 128-byte row line
+    const int64_t line_spread = (nsamps - nout) * line_ch / nchans;
+    const bool wide_groups = exact && cpg >= chunk && nchans >= line_ch && nchans % line_ch == 0;
+    if (wide_groups && line_spread <= T) {
+        cpg = chunk;
+        ilv = line_ch / chunk;
+        const int64_t hgroups = std::max<int64_t>(1, spp::ceil_div(groups, ilv));
+        nchunk = spp::ceil_div(nchans, line_ch * hgroups);
+        groups = ilv * spp::ceil_div(nchans, line_ch * nchunk);
+    } else if (wide_groups) {
+        cpg = chunk;
+        groups = spp::ceil_div(nchans, chunk);
+    } else {
+        groups = spp::ceil_div(nchans, cpg);
+    }
+    const int use_atomic = groups > 1 ? 1 : 0;
+    if (use_atomic) {
+        const int rc = fill_f32(out + out_start, nout, st);
+        if (rc) return rc;
+    }
+    return spp::launch_gulp_swar(in, nbits, big != 0, delays, out, nchans, nsamps, nout, out_start, row_bytes, cpg,
+                                 groups, ilv, nchunk, use_atomic, st);
+}
+
 template <int MODE>
 int launch_int(const uint8_t* in, int nbits, int big, const int32_t* delays, const int32_t* cts,
                float* out, int64_t nchans, int64_t nsamps, int64_t nout, int64_t out_start,
                int64_t nsubs, int64_t out_size, hipStream_t st) {
+    if (MODE == 0 && spp::gulp_swar_applies(in, nbits, nchans))
+        return launch_swar(in, nbits, big, delays, out, nchans, nsamps, nout, out_start, st);
     const int64_t row_bytes = nchans * nbits / 8;
     const int64_t bx = spp::ceil_div(nout, DD_T);
     SPP_REQUIRE(bx <= 0x7fffffffLL, "gulp too long");

@@ -1,0 +1,100 @@
+"""GPU parity of the word-parallel one-DM kernel (k_gulp_swar), the spp_dedisperse path for
+rows of whole dwords, against the C oracle, bit for bit.
+
+Cases beyond the operator and Filterbank tests in test_gpu_parity.py: delay patterns whose
+spread inside one stage (16 bytes of a row) or inside one dword column exceeds what one pass
+stages (random delays up to 3000 samples: passes of single channels), negative delays
+(numba's wraparound of a negative row index), the 2-bit 4096-channel gulp at DM 0, 100 and
+1000 (line-interleaved groups and one-chunk groups), channel counts that exercise the
+16-bit counter widening, and a stream whose packed rows are not whole dwords (the
+byte-staged kernel) beside one that is.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import c_oracle
+from oracle import spp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(samples, nbits, nchans, nsamps, delays, *, offset=0):
+    from sigpyproc3_amd import _lib
+
+    order = O.DEFAULT_BITORDER.get(nbits, "big")
+    packed = samples if nbits == 8 else O.pack(samples, nbits, order)
+    buf = np.zeros(packed.size + 8, np.uint8)
+    buf[offset : offset + packed.size] = packed
+    md = int(np.abs(delays).max())
+    dev = _lib.to_device(buf)
+    d_dev = _lib.to_device(delays.astype(np.int32))
+    out = _lib.zeros((nsamps - md,), "float32")
+    _lib.call("spp_dedisperse", _lib.ptr(dev[offset:]), nbits, 1 if order == "big" else 0, nchans, nsamps,
+              _lib.ptr(d_dev), md, _lib.ptr(out), 0, _lib.stream_handle())
+    ref = np.zeros(nsamps - md, np.float32)
+    c_oracle.dedisperse(samples, ref, delays.astype(np.int32), nchans, nsamps, 0)
+    return out.cpu().numpy(), ref
+
+
+@pytest.mark.parametrize("nbits", [1, 2, 4, 8])
+def test_random_wide_delays(nbits):
+    rng = np.random.default_rng(nbits)
+    nchans, nsamps = 256, 12000
+    samples = rng.integers(0, 1 << nbits, nsamps * nchans, dtype=np.uint8)
+    delays = rng.integers(0, 3000, nchans)
+    print(f"{nbits}-bit, random delays up to {delays.max()}", flush=True)
+    got, ref = _run(samples, nbits, nchans, nsamps, delays)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("nbits", [1, 2, 4, 8])
+def test_negative_delays_wrap(nbits):
+    rng = np.random.default_rng(10 + nbits)
+    nchans, nsamps = 128, 9000
+    samples = rng.integers(0, 1 << nbits, nsamps * nchans, dtype=np.uint8)
+    delays = rng.integers(-400, 400, nchans)
+    print(f"{nbits}-bit, delays in [-400, 400)", flush=True)
+    got, ref = _run(samples, nbits, nchans, nsamps, delays)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dm", [0.0, 100.0, 1000.0])
+def test_2bit_4096_channel_gulp(dm):
+    rng = np.random.default_rng(int(dm))
+    nchans, nsamps = 4096, 1 << 16
+    samples = rng.integers(0, 4, nsamps * nchans, dtype=np.uint8)
+    f = O.chan_freqs(nchans, -300.0 / nchans, 1550.0)
+    delays = O.compute_dmdelays(f, dm, 64e-6, 1550.0).astype(np.int32)
+    print(f"2-bit 4096 ch x 2^16, DM {dm}: max delay {delays.max()}", flush=True)
+    got, ref = _run(samples, 2, nchans, nsamps, delays)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize(("nbits", "nchans"), [(2, 2048), (4, 2048), (8, 1024), (1, 4096)])
+def test_counter_widening_many_channels(nbits, nchans):
+    """All samples at their maximum over many channels in few blocks: every counter level
+    carries its largest sums (a short stream keeps one channel group per time tile)."""
+    nsamps = 3000
+    samples = np.full(nsamps * nchans, (1 << nbits) - 1, dtype=np.uint8)
+    samples[::7] = 0
+    delays = (np.arange(nchans) // 64).astype(np.int32)
+    print(f"{nbits}-bit, {nchans} ch at full scale", flush=True)
+    got, ref = _run(samples, nbits, nchans, nsamps, delays)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("nbits", [1, 2, 4, 8])
+def test_unaligned_stream_and_odd_rows(nbits):
+    rng = np.random.default_rng(20 + nbits)
+    nsamps = 7000
+    for nchans, offset in [(96, 1), (72 if nbits < 8 else 36, 0)]:  # misaligned start; rows not whole dwords
+        if (nchans * nbits) % 8:
+            continue
+        samples = rng.integers(0, 1 << nbits, nsamps * nchans, dtype=np.uint8)
+        delays = np.sort(rng.integers(0, 500, nchans))[::-1].copy()
+        print(f"{nbits}-bit, {nchans} ch, offset {offset}", flush=True)
+        got, ref = _run(samples, nbits, nchans, nsamps, delays, offset=offset)
+        np.testing.assert_array_equal(got, ref)
