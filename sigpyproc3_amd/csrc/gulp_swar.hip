@@ -16,7 +16,9 @@
 // Work per sample staged: 1/F of a dword load and of an LDS store and about 2 log2(F)/F
 // transpose ops. Per channel and F outputs: one 8-byte LDS read, one funnel shift and five
 // adds and masks. For 2-bit data (F = 16) that is about 0.5 ops per sample staged and 0.45
-// per output-channel.
+// per output-channel. Time tiles are 2048 outputs (1/2-bit) or 1024 (4/8-bit): long enough
+// that one channel group per 16-byte row chunk takes a whole line of work, so the groups of
+// a 128-byte row line can run side by side (the line interleave) and share it in L2.
 //
 // A stage's channels whose delays span more than SPREAD samples are split into passes
 // (consecutive columns, or runs of one column's channels), each staging its own row window.
@@ -29,18 +31,9 @@
 namespace {
 
 constexpr int GS_THREADS = 256;
-#ifndef GS_STAGE_DWORDS
-#define GS_STAGE_DWORDS 4
-#endif
-constexpr int GS_S = GS_STAGE_DWORDS;  // dword columns per stage
-constexpr int GS_LOGS = GS_S == 8 ? 3 : (GS_S == 4 ? 2 : 1);
-#ifndef GS_RMAX_ROWS
-#define GS_RMAX_ROWS 2560
-#endif
-#ifndef GS_TILE_12
-#define GS_TILE_12 2048
-#endif
-constexpr int GS_RMAX = GS_RMAX_ROWS;  // rows one pass stages at most
+constexpr int GS_S = 4;     // dword columns per stage (16 bytes of each row)
+constexpr int GS_LOGS = 2;  // log2(GS_S)
+constexpr int GS_RMAX = 2560;  // rows one pass stages at most
 // Word 3 of a raw buffer resource on gfx9-family GPUs: 32-bit data format, no swizzle.
 constexpr int kBufferFlags = 0x00020000;
 
@@ -55,14 +48,15 @@ constexpr int series_pitch(int f, int nrb) {
 template <int NBITS>
 struct Geo {
     static constexpr int F = 32 / NBITS;                                    // samples per dword
-    static constexpr int T = NBITS == 8 ? 512 : (NBITS == 4 ? 1024 : (NBITS == 2 ? GS_TILE_12 : 2048));  // outputs per block
-    static constexpr int G = T / F;                                         // output groups
+    static constexpr int T = (NBITS == 8 || NBITS == 4) ? 1024 : 2048;     // outputs per block
+    static constexpr int NG = T / F > GS_THREADS ? T / F / GS_THREADS : 1;  // output groups per thread
+    static constexpr int G = T / F / NG;                                    // threads per partition
     static constexpr int H = GS_THREADS / G;                                // channel partitions
     static constexpr int NRB = GS_RMAX / F;                                 // series dwords used
     static constexpr int PITCH = series_pitch(F, NRB);
     static constexpr int SPREAD = GS_RMAX - T - 2 * F;                      // per pass
     static constexpr int CH = GS_S * F;                                     // channels per stage
-    static_assert(G * H == GS_THREADS && G >= 64, "a wave's lanes share one channel partition");
+    static_assert(G * H == GS_THREADS && G >= 64 && G * NG * F == T, "a wave's lanes share one channel partition");
 };
 
 constexpr uint32_t swap_mask(int s) {
@@ -251,7 +245,7 @@ __global__ __launch_bounds__(GS_THREADS, 3) void k_gulp_swar(
     int64_t nsamps, int64_t nout, int64_t out_start, int64_t row_bytes, int64_t chans_per_group, int64_t ngroups,
     int64_t nblocks, int64_t ilv, int64_t nchunk, int use_atomic) {
     using Gm = Geo<NBITS>;
-    constexpr int F = Gm::F, T = Gm::T, G = Gm::G, H = Gm::H, PITCH = Gm::PITCH, CH = Gm::CH;
+    constexpr int F = Gm::F, T = Gm::T, G = Gm::G, H = Gm::H, NG = Gm::NG, PITCH = Gm::PITCH, CH = Gm::CH;
     constexpr int UNITS = F >= 32 ? 1 : 2;  // staging units in flight per thread
     __shared__ uint32_t series[CH * PITCH];
     __shared__ __align__(16) int32_t sdel[CH];
@@ -267,7 +261,7 @@ __global__ __launch_bounds__(GS_THREADS, 3) void k_gulp_swar(
     const int64_t t0 = (logical / ngroups) * T;
     const int64_t grp = logical % ngroups;
     const int64_t gl = grp % ilv, gh = grp / ilv;
-    SwarAcc<NBITS> acc;
+    SwarAcc<NBITS> acc[NG];  // output groups g + G i, i < NG
 
     // The block's stages in order: chunk m of chans_per_group channels (a multiple of CH),
     // CH channels at a time; (m, j) is stage j of chunk m.
@@ -291,7 +285,7 @@ __global__ __launch_bounds__(GS_THREADS, 3) void k_gulp_swar(
         }
         // Staging: units (column q, row block b) -> rows rbase + F b + k, k < F.
         const int nqp = q1 - q0;
-        const int nrb = G + (dmax - dmin + F - 1) / F + 1;
+        const int nrb = T / F + (dmax - dmin + F - 1) / F + 1;
         const int units = nqp * nrb;
         const int64_t rbase = t0 + dmin;
         const int64_t col0 = (c / F + q0) * 4;  // byte offset of the pass's first column
@@ -344,27 +338,36 @@ __global__ __launch_bounds__(GS_THREADS, 3) void k_gulp_swar(
                 if (u + i * GS_THREADS < units) store_unit(u + i * GS_THREADS, w[i]);
         }
         __syncthreads();
-        // Summing: this thread's F outputs t0 + F g + j over its share of the pass's channels
-        // (wave-uniform: a wave's lanes share h), four channels' reads in flight at a time.
+        // Summing: this thread's outputs t0 + F (g + G i) + j over its share of the pass's
+        // channels (wave-uniform: a wave's lanes share h), four channels' reads in flight at
+        // a time.
         const int len = c1 - c0;
         const int k0 = c0 + (len * h) / H, k1 = c0 + (len * (h + 1)) / H;
         int k = k0;
         for (; k + 4 <= k1; k += 4) {
             int2 ci[4];
-            uint32_t lo[4], hi[4];
+            uint32_t lo[4][NG], hi[4][NG];
 #pragma unroll
             for (int j = 0; j < 4; ++j) ci[j] = cinfo[k + j];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                lo[j] = series[ci[j].x + g];
-                hi[j] = series[ci[j].x + g + 1];
-            }
+            for (int j = 0; j < 4; ++j)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc.add(__builtin_amdgcn_alignbit(hi[j], lo[j], static_cast<uint32_t>(ci[j].y)));
+                for (int i = 0; i < NG; ++i) {
+                    lo[j][i] = series[ci[j].x + g + G * i];
+                    hi[j][i] = series[ci[j].x + g + G * i + 1];
+                }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int i = 0; i < NG; ++i)
+                    acc[i].add(__builtin_amdgcn_alignbit(hi[j][i], lo[j][i], static_cast<uint32_t>(ci[j].y)));
         }
         for (; k < k1; ++k) {
             const int2 ci = cinfo[k];
-            acc.add(__builtin_amdgcn_alignbit(series[ci.x + g + 1], series[ci.x + g], static_cast<uint32_t>(ci.y)));
+#pragma unroll
+            for (int i = 0; i < NG; ++i)
+                acc[i].add(__builtin_amdgcn_alignbit(series[ci.x + g + G * i + 1], series[ci.x + g + G * i],
+                                                     static_cast<uint32_t>(ci.y)));
         }
     };
 
@@ -436,11 +439,14 @@ __global__ __launch_bounds__(GS_THREADS, 3) void k_gulp_swar(
         have = haven;
     }
     // The H partitions' sums of each output, then one store (or float atomic) per output.
-    uint32_t o[F];
-    acc.finish(o);
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < F; ++j) series[h * T + F * g + j] = o[j];
+    for (int i = 0; i < NG; ++i) {
+        uint32_t o[F];
+        acc[i].finish(o);
+#pragma unroll
+        for (int j = 0; j < F; ++j) series[h * T + F * (g + G * i) + j] = o[j];
+    }
     __syncthreads();
     float* dst = out + out_start;
     for (int t = tid; t < T; t += GS_THREADS) {
@@ -474,8 +480,7 @@ int64_t gulp_swar_tile(int nbits) {
 int64_t gulp_swar_stage_channels(int nbits) { return int64_t(GS_S) * (32 / nbits); }
 
 bool gulp_swar_applies(const void* in, int nbits, int64_t nchans) {
-    // 8-bit samples gain little from 4-sample words: the byte-staged kernel keeps them
-    if (nbits != 1 && nbits != 2 && nbits != 4) return false;
+    if (nbits != 1 && nbits != 2 && nbits != 4 && nbits != 8) return false;
     return (nchans * nbits) % 32 == 0 && reinterpret_cast<uintptr_t>(in) % 4 == 0;
 }
 

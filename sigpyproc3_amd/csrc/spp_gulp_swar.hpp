@@ -8,8 +8,8 @@
 
 namespace spp {
 
-// True when the SWAR kernel takes this launch: rows of whole dwords on a 4-byte aligned
-// stream, and at most swar_max_channels(nbits) channels per block.
+// True when the SWAR kernel takes this launch: 1/2/4/8-bit rows of whole dwords on a 4-byte
+// aligned stream.
 bool gulp_swar_applies(const void* in, int nbits, int64_t nchans);
 
 // out[out_start + t] (t < nout) = sum over channels of x[t + delays[c]][c], with x[row] for
