@@ -151,9 +151,9 @@ extern "C" int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int
         i = j;
     }
     p->use32 = static_cast<int64_t>(nchans) * maxv > 65535;
-    // 8 waves x 2 DM rows x 12 channels per stage at 6 waves per SIMD: 48 KB of LDS, three
-    // blocks per CU (nibble data: variant 3, byte data: variant 0); tunable per plan
-    p->variant = nbits <= 2 ? 3 : 0;
+    // Nibble data: 16 waves x 1 DM row x 16 channels per stage at 8 waves per SIMD (variant 4);
+    // byte data: 8 waves x 2 DM rows x 12 channels at 6 waves per SIMD (variant 0); tunable
+    p->variant = nbits <= 2 ? 4 : 0;
     p->gpitch = ((nsamps + 63) / 64) * 64;
     const int64_t ntiles = static_cast<int64_t>(p->tile_dm0.size());
     int64_t off = 0;
@@ -169,7 +169,7 @@ extern "C" int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int
     return SPP_OK;
 }
 
-extern "C" int spp_sweep_variant_count(void) { return 4; }
+extern "C" int spp_sweep_variant_count(void) { return 6; }
 
 extern "C" int spp_sweep_plan_set_variant(spp_sweep_plan* p, int variant) {
     spp::clear_error();
@@ -292,10 +292,10 @@ int launch_sweep_v(const spp_sweep_plan* p, const Extent& e, char* ws, float* pl
 
 // Block shapes: (DM rows per wave, waves, channels per stage, min waves per SIMD); all tile
 // SW_DT rows and take a multiple of 4 channels per stage (one packed offset word per 4).
-// Nibble data (1/2-bit): 20 channels per stage give 5-channel nibble groups for 2-bit data.
-// 6 waves per SIMD (<= 80 VGPRs: two accumulators live in scratch, touched only at the byte ->
-// 16-bit widenings, the rest of the spills sit in the epilogue) with 12 channels per stage
-// (48 KB of LDS) fits three blocks per CU and is the fastest shape measured.
+// The kernel is bound by its LDS reads. One DM row per wave in 16-wave blocks holds the
+// accumulators of one row (about 58 VGPRs, no spills) and so runs 8 waves per SIMD; with 16
+// channels per stage (64 KB of LDS, two blocks per CU) it is the fastest shape measured for
+// nibble data and on a par with the 2 x 8 x 12 shape for byte data.
 template <int NBITS, bool USE32>
 int launch_sweep(const spp_sweep_plan* p, const Extent& e, char* ws, float* plane, int64_t ld,
                  hipStream_t st) {
@@ -304,6 +304,8 @@ int launch_sweep(const spp_sweep_plan* p, const Extent& e, char* ws, float* plan
             case 1: return launch_sweep_v<NBITS, USE32, 2, 8, 20, 4>(p, e, ws, plane, ld, st);
             case 2: return launch_sweep_v<NBITS, USE32, 2, 8, 8, 6>(p, e, ws, plane, ld, st);
             case 3: return launch_sweep_v<NBITS, USE32, 2, 8, 12, 6>(p, e, ws, plane, ld, st);
+            case 4: return launch_sweep_v<NBITS, USE32, 1, 16, 16, 8>(p, e, ws, plane, ld, st);
+            case 5: return launch_sweep_v<NBITS, USE32, 1, 16, 12, 8>(p, e, ws, plane, ld, st);
             default: return launch_sweep_v<NBITS, USE32, 2, 8, 16, 4>(p, e, ws, plane, ld, st);
         }
     } else {
@@ -311,6 +313,8 @@ int launch_sweep(const spp_sweep_plan* p, const Extent& e, char* ws, float* plan
             case 1: return launch_sweep_v<NBITS, USE32, 2, 8, 16, 4>(p, e, ws, plane, ld, st);
             case 2: return launch_sweep_v<NBITS, USE32, 4, 4, 8, 3>(p, e, ws, plane, ld, st);
             case 3: return launch_sweep_v<NBITS, USE32, 2, 8, 8, 4>(p, e, ws, plane, ld, st);
+            case 4: return launch_sweep_v<NBITS, USE32, 1, 16, 16, 8>(p, e, ws, plane, ld, st);
+            case 5: return launch_sweep_v<NBITS, USE32, 1, 16, 12, 8>(p, e, ws, plane, ld, st);
             default: return launch_sweep_v<NBITS, USE32, 2, 8, 12, 6>(p, e, ws, plane, ld, st);
         }
     }

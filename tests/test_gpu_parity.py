@@ -214,7 +214,7 @@ def test_sweep_wide_accumulators(rng):
     _sweep_case(rng, 5000, 1024, 8, np.arange(0, 20.0), -0.3, 64e-6, variants="all")
     # 2-bit above 21845 channels and 1-bit above 65535 channels overflow 16-bit lanes
     _sweep_case(rng, 3000, 22000, 2, np.array([0.0, 1.0, 2.0]), -0.01, 64e-6, variants="all")
-    _sweep_case(rng, 2100, 65600, 1, np.array([0.0, 1.0]), -0.004, 64e-6, variants=(1, 3))
+    _sweep_case(rng, 2100, 65600, 1, np.array([0.0, 1.0]), -0.004, 64e-6, variants=(1, 3, 4, 5))
 
 
 @pytest.mark.parametrize("nbits", [2, 8])
