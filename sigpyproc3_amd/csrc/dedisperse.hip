@@ -379,6 +379,7 @@ int fill_f32(float* p, int64_t n, hipStream_t st) {
 int launch_swar(const uint8_t* in, int nbits, int big, const int32_t* delays, float* out, int64_t nchans,
                 int64_t nsamps, int64_t nout, int64_t out_start, hipStream_t st) {
     const int64_t row_bytes = nchans * nbits / 8;
+    const bool compact = spp::gulp_swar_compact(nbits, nchans, nsamps - nout);
     const int64_t T = spp::gulp_swar_tile(nbits);
     const int64_t chunk = spp::gulp_swar_stage_channels(nbits);  // channels of one stage
     const int64_t bx = spp::ceil_div(nout, T);
@@ -414,7 +415,7 @@ int launch_swar(const uint8_t* in, int nbits, int big, const int32_t* delays, fl
         if (rc) return rc;
     }
     return spp::launch_gulp_swar(in, nbits, big != 0, delays, out, nchans, nsamps, nout, out_start, row_bytes, cpg,
-                                 groups, ilv, nchunk, use_atomic, st);
+                                 groups, ilv, nchunk, use_atomic, compact, st);
 }
 
 template <int MODE>

@@ -33,7 +33,7 @@ namespace {
 constexpr int GS_THREADS = 256;
 constexpr int GS_S = 4;     // dword columns per stage (16 bytes of each row)
 constexpr int GS_LOGS = 2;  // log2(GS_S)
-constexpr int GS_RMAX = 2560;  // rows one pass stages at most
+constexpr int GS_RMAX = 2560;  // rows one pass stages at most (the compact form: fewer, below)
 // Word 3 of a raw buffer resource on gfx9-family GPUs: 32-bit data format, no swizzle.
 constexpr int kBufferFlags = 0x00020000;
 
@@ -45,18 +45,23 @@ constexpr int series_pitch(int f, int nrb) {
     return nrb + 1;
 }
 
-template <int NBITS>
+// COMPACT (4/8-bit data): a series buffer of half the rows, about 21 KB, so that twice as many
+// blocks share a CU (these kernels are bound by LDS size, not registers) when the stages' delay
+// spreads are small. 1/2-bit blocks are register-bound: more LDS room buys them nothing.
+template <int NBITS, bool COMPACT = false>
 struct Geo {
     static constexpr int F = 32 / NBITS;                                    // samples per dword
     static constexpr int T = (NBITS == 8 || NBITS == 4) ? 1024 : 2048;     // outputs per block
+    static constexpr int RMAX = COMPACT ? 1280 : GS_RMAX;
     static constexpr int NG = T / F > GS_THREADS ? T / F / GS_THREADS : 1;  // output groups per thread
     static constexpr int G = T / F / NG;                                    // threads per partition
     static constexpr int H = GS_THREADS / G;                                // channel partitions
-    static constexpr int NRB = GS_RMAX / F;                                 // series dwords used
+    static constexpr int NRB = RMAX / F;                                    // series dwords used
     static constexpr int PITCH = series_pitch(F, NRB);
-    static constexpr int SPREAD = GS_RMAX - T - 2 * F;                      // per pass
+    static constexpr int SPREAD = RMAX - T - 2 * F;                         // per pass
     static constexpr int CH = GS_S * F;                                     // channels per stage
     static_assert(G * H == GS_THREADS && G >= 64 && G * NG * F == T, "a wave's lanes share one channel partition");
+    static_assert(!COMPACT || NBITS >= 4, "compact form for byte-wide and nibble-wide samples only");
 };
 
 constexpr uint32_t swap_mask(int s) {
@@ -239,12 +244,12 @@ struct SwarAcc<8> {  // 16-bit fields (257 channels), 32-bit
     }
 };
 
-template <int NBITS, bool BIG>
+template <int NBITS, bool BIG, bool COMPACT>
 __global__ __launch_bounds__(GS_THREADS, 3) void k_gulp_swar(
     const uint8_t* __restrict__ in, const int32_t* __restrict__ delays, float* __restrict__ out, int64_t nchans,
     int64_t nsamps, int64_t nout, int64_t out_start, int64_t row_bytes, int64_t chans_per_group, int64_t ngroups,
     int64_t nblocks, int64_t ilv, int64_t nchunk, int use_atomic) {
-    using Gm = Geo<NBITS>;
+    using Gm = Geo<NBITS, COMPACT>;
     constexpr int F = Gm::F, T = Gm::T, G = Gm::G, H = Gm::H, NG = Gm::NG, PITCH = Gm::PITCH, CH = Gm::CH;
     constexpr int UNITS = F >= 32 ? 1 : 2;  // staging units in flight per thread
     __shared__ uint32_t series[CH * PITCH];
@@ -477,6 +482,14 @@ int64_t gulp_swar_tile(int nbits) {
     return nbits == 8 ? Geo<8>::T : (nbits == 4 ? Geo<4>::T : (nbits == 2 ? Geo<2>::T : Geo<1>::T));
 }
 
+bool gulp_swar_compact(int nbits, int64_t nchans, int64_t max_delay) {
+    if (nbits < 4 || nchans <= 0) return false;
+    const int64_t spread = nbits == 8 ? Geo<8, true>::SPREAD : Geo<4, true>::SPREAD;
+    // twice the mean delay spread of a stage's channels (the widest stages of a dispersion
+    // sweep run 1.4-1.6 times the mean)
+    return 2 * max_delay * gulp_swar_stage_channels(nbits) <= spread * nchans;
+}
+
 int64_t gulp_swar_stage_channels(int nbits) { return int64_t(GS_S) * (32 / nbits); }
 
 bool gulp_swar_applies(const void* in, int nbits, int64_t nchans) {
@@ -486,23 +499,27 @@ bool gulp_swar_applies(const void* in, int nbits, int64_t nchans) {
 
 int launch_gulp_swar(const uint8_t* in, int nbits, bool big, const int32_t* delays, float* out, int64_t nchans,
                      int64_t nsamps, int64_t nout, int64_t out_start, int64_t row_bytes, int64_t chans_per_group,
-                     int64_t groups, int64_t ilv, int64_t nchunk, int use_atomic, hipStream_t st) {
+                     int64_t groups, int64_t ilv, int64_t nchunk, int use_atomic, bool compact, hipStream_t st) {
     SPP_REQUIRE(chans_per_group * nchunk <= swar_max_channels(nbits), "too many channels per block");
+    SPP_REQUIRE(!compact || nbits >= 4, "the compact form is for 4- and 8-bit data");
     const int64_t bx = ceil_div(nout, gulp_swar_tile(nbits));
     const int64_t nblocks = bx * groups;
     SPP_REQUIRE(nblocks <= 0x7ffffff0LL, "gulp too long");
     const dim3 grid(static_cast<unsigned>(ceil_div(nblocks, int64_t(8)) * 8));
-#define SPP_SWAR(NB, BG)                                                                                   \
-    hipLaunchKernelGGL((k_gulp_swar<NB, BG>), grid, dim3(GS_THREADS), 0, st, in, delays, out, nchans, nsamps, \
+#define SPP_SWAR(NB, BG, CP)                                                                                   \
+    hipLaunchKernelGGL((k_gulp_swar<NB, BG, CP>), grid, dim3(GS_THREADS), 0, st, in, delays, out, nchans, nsamps, \
                        nout, out_start, row_bytes, chans_per_group, groups, nblocks, ilv, nchunk, use_atomic)
-    switch (nbits * 2 + (big ? 1 : 0)) {
-        case 2: SPP_SWAR(1, false); break;
-        case 3: SPP_SWAR(1, true); break;
-        case 4: SPP_SWAR(2, false); break;
-        case 5: SPP_SWAR(2, true); break;
-        case 8: SPP_SWAR(4, false); break;
-        case 9: SPP_SWAR(4, true); break;
-        default: SPP_SWAR(8, false); break;
+    switch (nbits * 4 + (big && nbits < 8 ? 2 : 0) + (compact ? 1 : 0)) {  // bit order: sub-byte samples only
+        case 4: SPP_SWAR(1, false, false); break;
+        case 6: SPP_SWAR(1, true, false); break;
+        case 8: SPP_SWAR(2, false, false); break;
+        case 10: SPP_SWAR(2, true, false); break;
+        case 16: SPP_SWAR(4, false, false); break;
+        case 17: SPP_SWAR(4, false, true); break;
+        case 18: SPP_SWAR(4, true, false); break;
+        case 19: SPP_SWAR(4, true, true); break;
+        case 33: SPP_SWAR(8, false, true); break;
+        default: SPP_SWAR(8, false, false); break;
     }
 #undef SPP_SWAR
     SPP_LAUNCHED("k_gulp_swar");

@@ -19,10 +19,14 @@ bool gulp_swar_applies(const void* in, int nbits, int64_t nchans);
 // zeroed output. Returns an SPP_* status.
 int launch_gulp_swar(const uint8_t* in, int nbits, bool big, const int32_t* delays, float* out, int64_t nchans,
                      int64_t nsamps, int64_t nout, int64_t out_start, int64_t row_bytes, int64_t chans_per_group,
-                     int64_t groups, int64_t ilv, int64_t nchunk, int use_atomic, hipStream_t st);
+                     int64_t groups, int64_t ilv, int64_t nchunk, int use_atomic, bool compact, hipStream_t st);
 
 // Outputs per block for nbits (the time tile).
 int64_t gulp_swar_tile(int nbits);
+
+// True when the compact form (a smaller series buffer: more blocks per CU, fewer rows per pass)
+// suits a launch whose largest delay is max_delay (4/8-bit data only).
+bool gulp_swar_compact(int nbits, int64_t nchans, int64_t max_delay);
 
 // Channels per stage for nbits (a stage is a fixed run of dwords of each row).
 int64_t gulp_swar_stage_channels(int nbits);

@@ -6,8 +6,9 @@ spread inside one stage (16 bytes of a row) or inside one dword column exceeds w
 stages (random delays up to 3000 samples: passes of single channels), negative delays
 (numba's wraparound of a negative row index), the 2-bit 4096-channel gulp at DM 0, 100 and
 1000 (line-interleaved groups and one-chunk groups), channel counts that exercise the
-16-bit counter widening, and a stream whose packed rows are not whole dwords (the
-byte-staged kernel) beside one that is.
+16-bit counter widening, the compact series buffer of 4/8-bit launches with a stage wider than
+one compact pass, and a stream whose packed rows are not whole dwords (the byte-staged kernel)
+beside one that is.
 """
 
 from __future__ import annotations
@@ -82,6 +83,21 @@ def test_counter_widening_many_channels(nbits, nchans):
     samples[::7] = 0
     delays = (np.arange(nchans) // 64).astype(np.int32)
     print(f"{nbits}-bit, {nchans} ch at full scale", flush=True)
+    got, ref = _run(samples, nbits, nchans, nsamps, delays)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("nbits", [4, 8])
+def test_compact_form_with_one_wide_stage(nbits):
+    """4/8-bit launches whose largest delay is small next to the channel count take the
+    compact series buffer (fewer rows per pass); one stage here still spans 600 samples, more
+    than a compact pass holds, and is split into passes."""
+    rng = np.random.default_rng(30 + nbits)
+    nchans, nsamps = 512, 9000
+    samples = rng.integers(0, 1 << nbits, nsamps * nchans, dtype=np.uint8)
+    delays = rng.integers(0, 40, nchans)
+    delays[64:96] = np.linspace(0, 600, 32).astype(np.int64)  # one wide stage of channels
+    print(f"{nbits}-bit, compact form, one stage spanning 600 samples", flush=True)
     got, ref = _run(samples, nbits, nchans, nsamps, delays)
     np.testing.assert_array_equal(got, ref)
 
