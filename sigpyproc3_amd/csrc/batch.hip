@@ -38,7 +38,7 @@ constexpr int BB_RUN = 32;                  // consecutive stage-1 time blocks p
 constexpr int BATCH_BCW = 32;               // packed bytes per row of a stage-1 item
 constexpr int BATCH_LOADS = 16;             // stage-1 staging loads in flight per thread
 constexpr uint32_t SPIN_LIMIT = 1u << 24;   // polls (about 0.2 us each) before a wait fails
-constexpr int NSHAPES = 4;
+constexpr int NSHAPES = 5;
 
 struct BInst {
     const uint8_t* packed;
@@ -270,7 +270,8 @@ __device__ __forceinline__ void next_item(const BArgs* A, BlockQueue* Q, int nsu
 
 template <int K, int WAVES, int CC, int MINW>
 __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW))) void k_batch(const BArgs* A) {
-    static_assert(WAVES * 64 == B_THREADS, "stage-1 items take the block's B_THREADS threads");
+    static_assert(WAVES * 64 == B_THREADS || WAVES * 64 == 2 * B_THREADS,
+                  "stage-1 items take B_THREADS threads of the block");
     constexpr int SWEEP_WORDS = 2 * CC * SW_LW;
     constexpr int BUILD_WORDS = (build_lds_bytes<1, BATCH_BCW>() + 3) / 4;  // IL = 8 stages the most
     __shared__ uint32_t lds[SWEEP_WORDS > BUILD_WORDS ? SWEEP_WORDS : BUILD_WORDS];
@@ -292,7 +293,13 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
         if (!uni(Q.skip)) {
             CInst& in = *(CInst*)(A->insts + uni(Q.inst));
             if (uni(Q.kind) == 0) {
-                run_build(in, uni(Q.a), uni(Q.b), reinterpret_cast<uint8_t*>(lds));
+                // The first B_THREADS threads stage the item; in a larger block the other waves
+                // take part in the stage-1 tile's one barrier only (tid < B_THREADS is the same
+                // in every lane of a wave).
+                if (WAVES * 64 == B_THREADS || tid < B_THREADS)
+                    run_build(in, uni(Q.a), uni(Q.b), reinterpret_cast<uint8_t*>(lds));
+                else
+                    __syncthreads();
             } else {
                 // Stage-2 unit: sub-tile sub of DM tile dt over time tile tt, the one-plan
                 // kernel's body for the instance's field width (included per case, so that it
@@ -367,11 +374,14 @@ template <> struct Shape<0> { static constexpr int K = 2, W = 8, CC = 12, MINW =
 template <> struct Shape<1> { static constexpr int K = 2, W = 8, CC = 16, MINW = 4; };
 template <> struct Shape<2> { static constexpr int K = 1, W = 8, CC = 8, MINW = 8; };
 template <> struct Shape<3> { static constexpr int K = 2, W = 8, CC = 8, MINW = 6; };
+// One DM row per wave in 16-wave blocks (as the one-plan kernel's nibble default): 64 KB of LDS,
+// two blocks per CU at 8 waves per SIMD.
+template <> struct Shape<4> { static constexpr int K = 1, W = 16, CC = 16, MINW = 8; };
 
 template <int S>
 int launch_shape(const BArgs* args, int64_t grid, hipStream_t st) {
     using Sh = Shape<S>;
-    hipLaunchKernelGGL((k_batch<Sh::K, Sh::W, Sh::CC, Sh::MINW>), dim3(static_cast<unsigned>(grid)), dim3(B_THREADS),
+    hipLaunchKernelGGL((k_batch<Sh::K, Sh::W, Sh::CC, Sh::MINW>), dim3(static_cast<unsigned>(grid)), dim3(Sh::W * 64),
                        0, st, args);
     SPP_LAUNCHED("k_batch");
     return SPP_OK;
@@ -380,7 +390,7 @@ int launch_shape(const BArgs* args, int64_t grid, hipStream_t st) {
 template <int S>
 int blocks_per_cu(int* n) {
     using Sh = Shape<S>;
-    SPP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_batch<Sh::K, Sh::W, Sh::CC, Sh::MINW>, B_THREADS, 0));
+    SPP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_batch<Sh::K, Sh::W, Sh::CC, Sh::MINW>, Sh::W * 64, 0));
     return SPP_OK;
 }
 
@@ -389,7 +399,8 @@ int shape_rows(int s) {
         case 0: return Shape<0>::K * Shape<0>::W;
         case 1: return Shape<1>::K * Shape<1>::W;
         case 2: return Shape<2>::K * Shape<2>::W;
-        default: return Shape<3>::K * Shape<3>::W;
+        case 3: return Shape<3>::K * Shape<3>::W;
+        default: return Shape<4>::K * Shape<4>::W;
     }
 }
 
@@ -398,7 +409,8 @@ int shape_cc(int s) {
         case 0: return Shape<0>::CC;
         case 1: return Shape<1>::CC;
         case 2: return Shape<2>::CC;
-        default: return Shape<3>::CC;
+        case 3: return Shape<3>::CC;
+        default: return Shape<4>::CC;
     }
 }
 
@@ -592,7 +604,8 @@ int launch(spp_batch* b, void* workspace, hipStream_t st) {
         case 0: return launch_shape<0>(args, b->grid, st);
         case 1: return launch_shape<1>(args, b->grid, st);
         case 2: return launch_shape<2>(args, b->grid, st);
-        default: return launch_shape<3>(args, b->grid, st);
+        case 3: return launch_shape<3>(args, b->grid, st);
+        default: return launch_shape<4>(args, b->grid, st);
     }
 }
 
@@ -603,7 +616,8 @@ int resident_grid(int shape, int64_t* grid) {
         case 0: rc = blocks_per_cu<0>(&per_cu); break;
         case 1: rc = blocks_per_cu<1>(&per_cu); break;
         case 2: rc = blocks_per_cu<2>(&per_cu); break;
-        default: rc = blocks_per_cu<3>(&per_cu); break;
+        case 3: rc = blocks_per_cu<3>(&per_cu); break;
+        default: rc = blocks_per_cu<4>(&per_cu); break;
     }
     if (rc != SPP_OK) return rc;
     const int64_t n = static_cast<int64_t>(std::max(per_cu, 1)) * spp::device_cu_count();

@@ -152,7 +152,8 @@ def test_bad_shape_and_empty_batch_are_refused(plans):
     lib = _lib.load()
     b = _batch(plans, 0)
     try:
-        assert lib.spp_batch_set_shape(b, 4, 1) == _lib.SPP_EINVAL
+        assert lib.spp_batch_set_shape(b, lib.spp_batch_shape_count(), 1) == _lib.SPP_EINVAL
+        assert lib.spp_batch_set_shape(b, -1, 1) == _lib.SPP_EINVAL
         assert lib.spp_batch_set_shape(b, 0, 0) == _lib.SPP_EINVAL
         with pytest.raises(ValueError, match="not the buffer"):
             _lib.call("spp_batch_run", b, ctypes.c_void_p(1 << 20), _info(b).workspace_bytes, None)

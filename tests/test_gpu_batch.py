@@ -72,7 +72,7 @@ def _compare(planes, ref):
         assert torch.equal(a, b), f"instance {k}: batch plane differs from the one-plan sweep"
 
 
-@pytest.mark.parametrize("shape,ntt", [(0, 1), (1, 2), (2, 1), (3, 4), (0, 3)])
+@pytest.mark.parametrize("shape,ntt", [(0, 1), (1, 2), (2, 1), (3, 4), (0, 3), (4, 1), (4, 2)])
 def test_batch_equals_one_plan_sweeps(case, shape, ntt):
     from sigpyproc3_amd.batch import SweepBatch
 
