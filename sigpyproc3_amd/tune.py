@@ -8,8 +8,8 @@ work runs on and keeps the fastest.
 Batches: ``tune_batch`` searches DM rows per item x channels per stage (the batch kernel's
 block shapes) x time tiles per item on the bound batch.
 
-Choices are cached in a JSON file keyed by the device and the exact work: nbits, nchans,
-nsamps and a hash of the delay table of every plan.
+Choices are cached in a JSON file keyed by the device, the number of block shapes the library
+offers, and the exact work: nbits, nchans, nsamps and a hash of the delay table of every plan.
 """
 
 from __future__ import annotations
@@ -51,12 +51,14 @@ def _plan_sig(plan) -> str:
 
 
 def _key(plan) -> str:
-    return f"{_device_name()}|{_plan_sig(plan)}"
+    # the shape count keeps a choice made by a library with another set of shapes from applying
+    return f"{_device_name()}|shapes={plan.variant_count()}|{_plan_sig(plan)}"
 
 
 def _batch_key(batch) -> str:
     h = hashlib.sha1("/".join(_plan_sig(p) for p in batch.plans).encode()).hexdigest()[:24]
-    return f"{_device_name()}|batch|n={len(batch.plans)}|rounds={int(batch.info.rounds)}|{h}"
+    return (f"{_device_name()}|batch|shapes={batch.shape_count()}|n={len(batch.plans)}"
+            f"|rounds={int(batch.info.rounds)}|{h}")
 
 
 def _load() -> dict:
