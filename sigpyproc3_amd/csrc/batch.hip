@@ -128,26 +128,27 @@ __device__ __forceinline__ int opaque_tid() {
     return t;
 }
 
-template <int NBITS, bool BIG>
+template <int NBITS, bool BIG, int NT>
 __device__ __forceinline__ void build_item(CInst& in, int64_t j0, int64_t b0, uint8_t* raw) {
     if (in.aligned)
-        build_tile<NBITS, BIG, true, BATCH_BCW, BATCH_LOADS>(glob(in.packed), in.nsamps, in.nchans, glob(in.G), in.gpitch,
-                                                in.glen, j0, b0, raw, opaque_tid());
+        build_tile<NBITS, BIG, true, BATCH_BCW, BATCH_LOADS, NT>(glob(in.packed), in.nsamps, in.nchans, glob(in.G),
+                                                                 in.gpitch, in.glen, j0, b0, raw, opaque_tid());
     else
-        build_tile<NBITS, BIG, false, BATCH_BCW, BATCH_LOADS>(glob(in.packed), in.nsamps, in.nchans, glob(in.G), in.gpitch,
-                                                 in.glen, j0, b0, raw, opaque_tid());
+        build_tile<NBITS, BIG, false, BATCH_BCW, BATCH_LOADS, NT>(glob(in.packed), in.nsamps, in.nchans, glob(in.G),
+                                                                  in.gpitch, in.glen, j0, b0, raw, opaque_tid());
 }
 
 // Stage-1 unit: words [j0, j0 + B_TJ) of the packed row bytes [b0, b0 + BATCH_BCW).
+template <int NT>
 __device__ __forceinline__ void run_build(CInst& in, int64_t j0, int64_t b0, uint8_t* raw) {
     switch (in.nbits * 2 + in.big) {
-        case 2: build_item<1, false>(in, j0, b0, raw); break;
-        case 3: build_item<1, true>(in, j0, b0, raw); break;
-        case 4: build_item<2, false>(in, j0, b0, raw); break;
-        case 5: build_item<2, true>(in, j0, b0, raw); break;
-        case 8: build_item<4, false>(in, j0, b0, raw); break;
-        case 9: build_item<4, true>(in, j0, b0, raw); break;
-        default: build_item<8, false>(in, j0, b0, raw); break;
+        case 2: build_item<1, false, NT>(in, j0, b0, raw); break;
+        case 3: build_item<1, true, NT>(in, j0, b0, raw); break;
+        case 4: build_item<2, false, NT>(in, j0, b0, raw); break;
+        case 5: build_item<2, true, NT>(in, j0, b0, raw); break;
+        case 8: build_item<4, false, NT>(in, j0, b0, raw); break;
+        case 9: build_item<4, true, NT>(in, j0, b0, raw); break;
+        default: build_item<8, false, NT>(in, j0, b0, raw); break;
     }
 }
 
@@ -271,7 +272,7 @@ __device__ __forceinline__ void next_item(const BArgs* A, BlockQueue* Q, int nsu
 template <int K, int WAVES, int CC, int MINW>
 __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW))) void k_batch(const BArgs* A) {
     static_assert(WAVES * 64 == B_THREADS || WAVES * 64 == 2 * B_THREADS,
-                  "stage-1 items take B_THREADS threads of the block");
+                  "stage-1 items take the block's threads (B_THREADS or twice that)");
     constexpr int SWEEP_WORDS = 2 * CC * SW_LW;
     constexpr int BUILD_WORDS = (build_lds_bytes<1, BATCH_BCW>() + 3) / 4;  // IL = 8 stages the most
     __shared__ uint32_t lds[SWEEP_WORDS > BUILD_WORDS ? SWEEP_WORDS : BUILD_WORDS];
@@ -293,13 +294,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(MINW
         if (!uni(Q.skip)) {
             CInst& in = *(CInst*)(A->insts + uni(Q.inst));
             if (uni(Q.kind) == 0) {
-                // The first B_THREADS threads stage the item; in a larger block the other waves
-                // take part in the stage-1 tile's one barrier only (tid < B_THREADS is the same
-                // in every lane of a wave).
-                if (WAVES * 64 == B_THREADS || tid < B_THREADS)
-                    run_build(in, uni(Q.a), uni(Q.b), reinterpret_cast<uint8_t*>(lds));
-                else
-                    __syncthreads();
+                run_build<WAVES * 64>(in, uni(Q.a), uni(Q.b), reinterpret_cast<uint8_t*>(lds));
             } else {
                 // Stage-2 unit: sub-tile sub of DM tile dt over time tile tt, the one-plan
                 // kernel's body for the instance's field width (included per case, so that it

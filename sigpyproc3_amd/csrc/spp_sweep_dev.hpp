@@ -69,8 +69,9 @@ constexpr int build_lds_bytes() {
 // per (row, word) and stores them as one 16-byte write per channel (a wave writes 512
 // contiguous bytes per channel). Only the first glen words of each channel row are written
 // (the sweep reads no further). One barrier, reached by every thread of the block. MAXB bounds
-// the staging loads in flight per thread (registers); tid is the thread's index in the block.
-template <int NBITS, bool BIG, bool ALIGNED, int BCW, int MAXB = 32>
+// the staging loads in flight per thread (registers); tid is the thread's index in the block
+// of NT threads (B_THREADS or a multiple).
+template <int NBITS, bool BIG, bool ALIGNED, int BCW, int MAXB = 32, int NT = B_THREADS>
 __device__ __forceinline__ void build_tile(const uint8_t* __restrict__ in, int64_t nsamps, int64_t nchans,
                                            uint32_t* __restrict__ G, int64_t gpitch, int64_t glen,
                                            int64_t j0, int64_t b0, uint8_t* raw, int tid) {
@@ -84,10 +85,10 @@ __device__ __forceinline__ void build_tile(const uint8_t* __restrict__ in, int64
     // on distinct banks.
     constexpr int RQ = ROWS / 4 + 1;
     constexpr int STRIDE = BCW + 4;
-    constexpr int NE = ROWS * QPR / B_THREADS;            // staged dwords per thread
+    constexpr int NE = ROWS * QPR / NT;                   // staged dwords per thread
     constexpr int BATCH = NE < MAXB ? NE : MAXB;          // loads in flight per thread
     static_assert(BCW % 16 == 0 && BCW <= 64, "16, 32, 48 or 64 byte columns");
-    static_assert(NE % BATCH == 0 && NE * B_THREADS == ROWS * QPR, "whole batches");
+    static_assert(NE >= 1 && NE % BATCH == 0 && NE * NT == ROWS * QPR, "whole batches");
     const int64_t row_bytes = nchans * NBITS / 8;
     // Stage: element e = (row r, dword q of the row's BCW bytes), loaded BATCH at a time.
 #pragma unroll 1
@@ -95,7 +96,7 @@ __device__ __forceinline__ void build_tile(const uint8_t* __restrict__ in, int64
         uint32_t v[BATCH];
 #pragma unroll
         for (int i = 0; i < BATCH; ++i) {
-            const int e = tid + B_THREADS * (e0 + i);
+            const int e = tid + NT * (e0 + i);
             const int r = e / QPR, q = e % QPR;
             const int64_t row = j0 + static_cast<int64_t>(r / B_TJ) * SW_S + (r % B_TJ);
             v[i] = 0;
@@ -114,7 +115,7 @@ __device__ __forceinline__ void build_tile(const uint8_t* __restrict__ in, int64
         }
 #pragma unroll
         for (int i = 0; i < BATCH; ++i) {
-            const int e = tid + B_THREADS * (e0 + i);
+            const int e = tid + NT * (e0 + i);
             const int r = e / QPR, q = e % QPR;
             *reinterpret_cast<uint32_t*>(&raw[((r & 3) * RQ + (r >> 2)) * STRIDE + 4 * q]) = v[i];
         }
@@ -123,7 +124,7 @@ __device__ __forceinline__ void build_tile(const uint8_t* __restrict__ in, int64
     const int qd = tid & 31;
     const int64_t j = j0 + 4 * qd;
     if (j >= glen) return;
-    for (int bc = tid >> 5; bc < BCW; bc += B_THREADS / 32) {
+    for (int bc = tid >> 5; bc < BCW; bc += NT / 32) {
         uint32_t w[PER][4];
 #pragma unroll
         for (int p = 0; p < PER; ++p)
