@@ -101,8 +101,9 @@ int spp_sweep_plan_create(const int32_t* table_host, int64_t ndm, int64_t nchans
 int spp_sweep_plan_destroy(spp_sweep_plan* plan);
 /* Block shapes of the stage-2 kernel (DM rows per wave x waves x channels per stage):
  * 1/2-bit data: 0 = 2x8x16, 1 = 2x8x20, 2 = 2x8x8 (6 waves per SIMD), 3 = 2x8x12 (6 waves per
- *               SIMD, default);
- * 4/8-bit data: 0 = 2x8x12 (6 waves per SIMD, default), 1 = 2x8x16, 2 = 4x4x8, 3 = 2x8x8.
+ *               SIMD), 4 = 1x16x16 (8 waves per SIMD, default), 5 = 1x16x12 (8 waves per SIMD);
+ * 4/8-bit data: 0 = 2x8x12 (6 waves per SIMD, default), 1 = 2x8x16, 2 = 4x4x8, 3 = 2x8x8,
+ *               4 = 1x16x16, 5 = 1x16x12 (8 waves per SIMD).
  * All give identical results; spp_sweep_plan_set_variant picks one for a plan (the
  * autotuner times them). */
 int spp_sweep_variant_count(void);
@@ -174,9 +175,10 @@ typedef struct {
 int spp_batch_create(const spp_sweep_plan* const* plans, int32_t n, int64_t g_budget_bytes,
                      spp_batch** batch);
 int spp_batch_destroy(spp_batch* batch);
-/* Block shapes (DM rows per item x channels per stage): 0 = 16 x 12 (default), 1 = 16 x 16,
- * 2 = 8 x 8, 3 = 16 x 8; each stage-2 item covers time_tiles_per_item consecutive time tiles
- * (2048 samples for 1/2-bit data, 1024 for 4/8-bit). All give identical planes. */
+/* Block shapes (DM rows per item x channels per stage): 0 = 16 x 12, 1 = 16 x 16, 2 = 8 x 8,
+ * 3 = 16 x 8 (8-wave blocks), 4 = 16 x 16 (16 waves of one DM row, default); each stage-2 item
+ * covers time_tiles_per_item consecutive time tiles (2048 samples for 1/2-bit data, 1024 for
+ * 4/8-bit). All give identical planes. */
 int spp_batch_shape_count(void);
 int spp_batch_set_shape(spp_batch* batch, int shape, int32_t time_tiles_per_item);
 int spp_batch_get_info(const spp_batch* batch, spp_batch_info* info);

@@ -423,7 +423,7 @@ int64_t tables_bytes(const spp_sweep_plan* p) {
 
 struct spp_batch {
     std::vector<const spp_sweep_plan*> plans;
-    int shape = 0;
+    int shape = 4;  // 16 waves of one DM row (Shape<4>): the fastest on the mixed 64-instance batch
     int ntt = 1;
     int64_t g_budget = 0;
     int nrounds = 0, npools = 1;

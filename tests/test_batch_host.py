@@ -75,7 +75,7 @@ def test_batch_info_struct_layout():
     assert [f[1] for f in fields] == [f[0] for f in _lib.BatchInfo._fields_]
 
 
-@pytest.mark.parametrize("shape", range(4))
+@pytest.mark.parametrize("shape", range(5))
 @pytest.mark.parametrize("ntt", [1, 3])
 def test_queue_item_counts(plans, shape, ntt):
     lib = _lib.load()
