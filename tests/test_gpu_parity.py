@@ -202,6 +202,21 @@ def test_sweep_vs_oracle(nbits, nchans, rng):
                 variants="all")
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_sweep_random_configs(seed):
+    """Seeded random sweeps: field width, a ragged channel count, stream length, an irregular
+    ascending DM list and the band, every block shape against the C oracle."""
+    r = np.random.default_rng(1000 + seed)
+    nbits = int(r.choice([1, 2, 4, 8]))
+    step = 8 // nbits if nbits < 8 else 1
+    nchans = int(r.integers(16, 700)) // step * step + step
+    nsamps = int(r.integers(6000, 30000))
+    dms = np.sort(r.uniform(0, 80, int(r.integers(3, 40))))
+    bw = float(r.uniform(50, 400))
+    print(f"seed {seed}: {nbits}-bit {nchans} ch {nsamps} samples, {dms.size} DMs, {bw:.0f} MHz", flush=True)
+    _sweep_case(r, nsamps, nchans, nbits, dms, -bw / nchans, 128e-6, fch1=1500.0, variants="all")
+
+
 def test_sweep_wide_spread_tiles(rng):
     # DM steps so large that tiles hold fewer than 16 rows; ragged last tile
     plan = _sweep_case(rng, 30000, 128, 2, np.arange(0, 2000, 90.0), -2.0, 64e-6)
