@@ -219,7 +219,7 @@ def test_sweep_random_configs(seed):
 
 def test_sweep_wide_spread_tiles(rng):
     # DM steps so large that tiles hold fewer than 16 rows; ragged last tile
-    plan = _sweep_case(rng, 30000, 128, 2, np.arange(0, 2000, 90.0), -2.0, 64e-6)
+    plan = _sweep_case(rng, 30000, 128, 2, np.arange(0, 2000, 90.0), -2.0, 64e-6, variants="all")
     assert plan.info.ndm_tiles > 2
 
 
