@@ -5,10 +5,12 @@
 // fit one G pool; each round has a stage-1 segment B_r (items: B_TJ words x BATCH_BCW packed
 // bytes of one instance) and a stage-2 segment S_r (items: D DM rows x ntt time tiles of one
 // instance). With R rounds the order is
-//   B_0, B_1, S_0, B_2, S_1, ..., B_{R-1}, S_{R-2}, S_{R-1}
+//   B_0, B_1, S_0, S_1, B_2, B_3, S_2, S_3, B_4, ...
 // and two pools alternate: S_r waits until every item of B_r is complete, B_r (r >= 2) until
-// every item of S_{r-2} is (it overwrites the pool S_{r-2} read). One round needs no pool
-// reuse. Every segment has 8 sub-queues, one per XCD (block b runs on XCD b % 8); a
+// every item of S_{r-2} is (it overwrites the pool S_{r-2} read). Each wait is on the segment
+// two places earlier, which has drained while the segment between ran, so blocks do not idle
+// at segment changes (with B_2 right after S_0 every block waited for S_0's last items). One
+// round needs no pool reuse. Every segment has 8 sub-queues, one per XCD (block b runs on XCD b % 8); a
 // sub-queue takes runs of consecutive time tiles (stage 2) or time blocks (stage 1) of every
 // instance, so the blocks that share G windows or staged rows share that XCD's L2. A block
 // drains its own sub-queue, then takes items from the others, then moves to the next
@@ -553,9 +555,12 @@ QueueTables queue_tables(const spp_batch* b) {
         t.segs.push_back(s);
     };
     add(0, 0);
-    for (int r = 0; r < R; ++r) {
-        if (r + 1 < R) add(0, r + 1);
+    if (R > 1) add(0, 1);
+    for (int r = 0; r < R; r += 2) {
         add(1, r);
+        if (r + 1 < R) add(1, r + 1);
+        if (r + 2 < R) add(0, r + 2);
+        if (r + 3 < R) add(0, r + 3);
     }
     for (int r = 0; r < R; ++r) {
         t.segs[seg_s[r]].dep = seg_b[r];
