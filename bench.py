@@ -2,7 +2,7 @@
 """Benchmark: the 2-bit, 4096-channel, 2^20-sample, 2048-DM (0..2047) dedispersion sweep.
 
 One step = one full sweep of the packed stream resident in HBM (stage 1 + stage 2 of
-``spp_sweep_execute``). Prints one JSON line (rank 0) with the driver's contract fields
+``spp_sweep_execute``). Prints one JSON line (rank 0): metric, value, unit, timing and config
 plus ``roofline`` (the dominant kernel, k_sweep, timed with HIP events on its stream)
 and ``cpu_baseline`` (the C/OpenMP restatement of kernels.dedisperse_par, timed on a
 bounded DM sample on this host and extrapolated per DM-trial sample).
