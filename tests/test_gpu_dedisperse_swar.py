@@ -1,5 +1,6 @@
 """GPU parity of the word-parallel one-DM kernel (k_gulp_swar), the spp_dedisperse path for
-rows of whole dwords, against the C oracle, bit for bit.
+rows of whole dwords, against the C oracle, bit for bit; and of spp_subband on irregular band
+maps.
 
 Cases beyond the operator and Filterbank tests in test_gpu_parity.py: delay patterns whose
 spread inside one stage (16 bytes of a row) or inside one dword column exceeds what one pass
@@ -114,3 +115,54 @@ def test_unaligned_stream_and_odd_rows(nbits):
         print(f"{nbits}-bit, {nchans} ch, offset {offset}", flush=True)
         got, ref = _run(samples, nbits, nchans, nsamps, delays, offset=offset)
         np.testing.assert_array_equal(got, ref)
+
+
+def _subband(samples, nbits, nchans, nsamps, delays, cts):
+    from sigpyproc3_amd import _lib
+
+    order = O.DEFAULT_BITORDER.get(nbits, "big")
+    packed = samples if nbits == 8 else O.pack(samples, nbits, order)
+    md = int(delays.max())
+    nsubs = int(cts.max()) + 1
+    dev = _lib.to_device(packed)
+    d_dev = _lib.to_device(delays.astype(np.int32))
+    c_dev = _lib.to_device(cts.astype(np.int32))
+    sub = _lib.zeros(((nsamps - md) * nsubs,), "float32")
+    _lib.call("spp_subband", _lib.ptr(dev), nbits, 1 if order == "big" else 0, nchans, nsamps, _lib.ptr(d_dev),
+              _lib.ptr(c_dev), md, nsubs, _lib.ptr(sub), int(sub.numel()), _lib.stream_handle())
+    ref = np.zeros((nsamps - md) * nsubs, np.float32)
+    c_oracle.subband(samples, ref, delays.astype(np.int32), cts.astype(np.int32), md, nchans, nsubs, nsamps)
+    return sub.cpu().numpy(), ref
+
+
+@pytest.mark.parametrize("nbits", [2, 4, 8])
+@pytest.mark.parametrize("kind", ["uneven", "interleaved"])
+def test_subband_band_maps(nbits, kind):
+    """spp_subband with uneven contiguous bands whose edges fall anywhere (inside dwords and
+    staged column groups), and with bands whose channels interleave (a band run ends every 37
+    channels), against the C oracle."""
+    rng = np.random.default_rng(50 + nbits + (0 if kind == "uneven" else 7))
+    nchans, nsamps = 1024, 20000
+    samples = rng.integers(0, 1 << nbits, nsamps * nchans, dtype=np.uint8)
+    f = O.chan_freqs(nchans, -300.0 / nchans, 1500.0)
+    delays = O.compute_dmdelays(f, 60.0, 64e-6, 1500.0).astype(np.int64)
+    if kind == "uneven":
+        edges = np.sort(rng.choice(np.arange(1, nchans), 11, replace=False))
+        cts = np.searchsorted(edges, np.arange(nchans), side="right")
+    else:
+        cts = (np.arange(nchans) // 37) % 6
+    print(f"{nbits}-bit subband, {kind} bands ({int(cts.max()) + 1})", flush=True)
+    got, ref = _subband(samples, nbits, nchans, nsamps, delays, cts)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_subband_full_scale_counters():
+    """2-bit samples at their maximum in wide bands: every band run's sum is the largest it
+    can be."""
+    nchans, nsamps = 2048, 6000
+    samples = np.full(nsamps * nchans, 3, dtype=np.uint8)
+    samples[::5] = 0
+    delays = (np.arange(nchans) // 32).astype(np.int64)
+    cts = np.arange(nchans) // 512
+    got, ref = _subband(samples, 2, nchans, nsamps, delays, cts)
+    np.testing.assert_array_equal(got, ref)
