@@ -293,7 +293,7 @@ int launch_sweep_v(const spp_sweep_plan* p, const Extent& e, char* ws, float* pl
 // Block shapes: (DM rows per wave, waves, channels per stage, min waves per SIMD); all tile
 // SW_DT rows and take a multiple of 4 channels per stage (one packed offset word per 4).
 // The kernel is bound by its LDS reads. One DM row per wave in 16-wave blocks holds the
-// accumulators of one row (about 58 VGPRs, no spills) and so runs 8 waves per SIMD; with 16
+// accumulators of one row (about 60 VGPRs, no spills) and so runs 8 waves per SIMD; with 16
 // channels per stage (64 KB of LDS, two blocks per CU) it is the fastest shape measured for
 // nibble data and on a par with the 2 x 8 x 12 shape for byte data.
 template <int NBITS, bool USE32>
